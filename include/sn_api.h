@@ -1,0 +1,217 @@
+/*
+ * sn_api.h -- C ABI of libsupernormal_amd.so, the MI355X (gfx950) kernels of
+ * SuperNormal's training hot path.
+ *
+ * Conventions (every entry point):
+ *   - All tensor arguments are DEVICE pointers to caller-owned, contiguous
+ *     buffers.  Nothing here allocates device memory unless the comment says
+ *     so; scratch space is passed in as `workspace` with a size queried from
+ *     the matching *_workspace_bytes() function.
+ *   - `stream` is a hipStream_t passed as void*; all work is enqueued on it,
+ *     nothing synchronises the device.
+ *   - Return value: 0 on success, a negative SN_E* code on failure.  The
+ *     message of the last failure on the calling host thread is returned by
+ *     sn_last_error().  No C++ exception crosses this boundary.
+ *   - dtypes follow the reference: packed_info int32 [n,2] (base, count),
+ *     ray_indices int64, all data float32.
+ *
+ * Reference paths are relative to the SuperNormal repository; NA/ is
+ * third_parties/nerfacc-0.3.5/nerfacc-0.3.5/ and csrc/ is NA/nerfacc/cuda/csrc/.
+ */
+#ifndef SN_API_H
+#define SN_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SN_OK 0
+#define SN_EINVAL (-1)  /* bad argument (shape, size, null pointer)       */
+#define SN_EHIP (-2)    /* a HIP runtime call failed                       */
+#define SN_ESPACE (-3)  /* workspace or capacity too small                 */
+
+/* Message of the last failure on this host thread ("" if none). */
+const char* sn_last_error(void);
+/* ABI version of this library (increments on any signature change). */
+int sn_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Packing: replaces nerfacc.pack_info (NA/nerfacc/pack.py:46-77).           */
+/* counts[r] = #{i : ray_indices[i] == r}; packed_info[r] = (exclusive       */
+/* cumsum of counts, count).  Indices outside [0, n_rays) are an error in the */
+/* reference (scatter_add index error); here they are ignored and counted in */
+/* the optional `n_bad` output (device int32, nullable).                     */
+/* ------------------------------------------------------------------------ */
+size_t sn_pack_info_workspace_bytes(int32_t n_rays);
+int sn_pack_info(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
+                 int32_t* packed_info, int32_t* n_bad, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* Exclusive scan of int32 counts into packed_info rows (base, count); also  */
+/* writes the grand total to *total (device int32, nullable).  This is the   */
+/* cumsum step of csrc/ray_marching.cu:257-258.                             */
+size_t sn_scan_workspace_bytes(int64_t n);
+int sn_counts_to_packed_info(const int32_t* counts, int32_t n, int32_t* packed_info,
+                             int32_t* total, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Patch-based render weights: replaces                                      */
+/*   _C.weight_from_alpha_patch_based_forward_naive  (csrc/render_weight.cu:502-543)  */
+/*   _C.weight_and_transmittance_..._forward_naive    (csrc/render_weight.cu:629-667)  */
+/*   _C.weight_from_alpha_patch_based_backward_naive (csrc/render_weight.cu:583-626)  */
+/*   _C.weight_and_transmittance_..._backward_naive   (csrc/render_weight.cu:669-712)  */
+/* alphas, weights, transmittance: float [n_samples, patch_size] (sample-     */
+/* major); patch p owns samples packed_info[p,0] .. +packed_info[p,1].       */
+/* w[s,k] = alpha[s,k] * T[s,k], T exclusive product of (1 - alpha) over the */
+/* patch's earlier samples, for each ray k of the patch independently.       */
+/* Samples not covered by any patch are left untouched (reference: empty).   */
+/* n_samples bounds every access: a patch whose [base, base+count) does not  */
+/* fit in [0, n_samples) is skipped.  transmittance (fwd) is nullable.       */
+/* Backward uses the division-free suffix recurrence                         */
+/*   g_alpha[j] = T[j] * (gw[j] - S[j]),  S[j] = gw[j+1]a[j+1] + (1-a[j+1])S[j+1] */
+/* which equals the reference's (gw T - accum)/max(1-a,1e-10) wherever that  */
+/* is finite; grad_T (nullable) adds d(sum gT*T)/d alpha.                    */
+/* ------------------------------------------------------------------------ */
+int sn_patch_weight_fwd(const int32_t* packed_info, int32_t n_patches, int32_t patch_size,
+                        int64_t n_samples, const float* alphas, float* weights,
+                        float* transmittance, void* stream);
+int sn_patch_weight_bwd(const int32_t* packed_info, int32_t n_patches, int32_t patch_size,
+                        int64_t n_samples, const float* alphas, const float* grad_weights,
+                        const float* grad_transmittance, float* grad_alphas, void* stream);
+
+/* Patch-based transmittance only: replaces                                  */
+/*   _C.transmittance_from_alpha_patch_based_forward_naive (csrc/render_transmittance.cu:147-176, */
+/*   with alphas[ray_id] in place of the reference's alphas[j], line 173)    */
+/* The backward is sn_patch_weight_bwd with grad_weights == NULL.            */
+int sn_patch_transmittance_fwd(const int32_t* packed_info, int32_t n_patches,
+                               int32_t patch_size, int64_t n_samples, const float* alphas,
+                               float* transmittance, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Visibility: replaces render_visibility (NA/nerfacc/vol_rendering.py:680-748) */
+/* with the per-ray sequential exclusive product of                         */
+/* csrc/render_transmittance.cu:85-112.  keep[i] = T[i] >= early_stop_eps    */
+/* && (alpha_thre <= 0 || alpha[i] >= alpha_thre).  keep is uint8 (bool).   */
+/* ------------------------------------------------------------------------ */
+int sn_visibility(const int32_t* packed_info, int32_t n_rays, int64_t n_samples,
+                  const float* alphas, float early_stop_eps, float alpha_thre, uint8_t* keep,
+                  void* stream);
+
+/* Stream compaction of marching samples by a keep mask (ray_marching.py:216-220). */
+/* Writes the kept (ray_indices, t_starts, t_ends) in order and the count to */
+/* *n_out (device int32).  Output capacity must be >= n.                    */
+size_t sn_compact_workspace_bytes(int64_t n);
+int sn_compact_samples(const uint8_t* keep, int32_t n, const int64_t* ray_indices,
+                       const float* t_starts, const float* t_ends, int64_t* out_ray_indices,
+                       float* out_t_starts, float* out_t_ends, int32_t* n_out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Ray marching with occupancy-grid skipping: replaces _C.ray_marching       */
+/* (csrc/ray_marching.cu:81-289), AABB contraction, cone_angle 0 or > 0.     */
+/* Pass 1 (count) writes num_steps[r]; pass 2 (fill) uses packed_info from   */
+/* sn_counts_to_packed_info.  `max_per_ray` > 0 truncates each ray after     */
+/* that many samples (0 = unbounded, the reference's behaviour).  Floating  */
+/* point follows the reference kernel with no FMA contraction.              */
+/* grid: uint8 [rx, ry, rz] (x-major, index x*ry*rz + y*rz + z).            */
+/* ------------------------------------------------------------------------ */
+int sn_ray_march_count(const float* rays_o, const float* rays_d, const float* t_min,
+                       const float* t_max, int32_t n_rays, const float* roi_aabb,
+                       const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
+                       float step_size, float cone_angle, int32_t max_per_ray,
+                       int32_t* num_steps, void* stream);
+int sn_ray_march_fill(const float* rays_o, const float* rays_d, const float* t_min,
+                      const float* t_max, int32_t n_rays, const float* roi_aabb,
+                      const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
+                      float step_size, float cone_angle, int32_t max_per_ray,
+                      const int32_t* packed_info, int64_t* ray_indices, float* t_starts,
+                      float* t_ends, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Patch accumulation: replaces accumulate_along_rays_patch_based            */
+/* (NA/nerfacc/vol_rendering.py:269-335): out[ray_indices[s], k, d] +=       */
+/* src[s, k, d] for src float [n, P, D].  `out` must be zeroed by the caller */
+/* (float atomics, like the reference's scatter_add_).                      */
+/* ------------------------------------------------------------------------ */
+int sn_scatter_add_rows(const int64_t* ray_indices, int64_t n, int32_t row_elems,
+                        const float* src, float* out, int32_t n_out_rows, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Occupancy grid (NA/nerfacc/grid.py:196-239, csrc/contraction.cu:36-61).   */
+/* sn_occ_cell_points: x[i] = ((coord(idx[i]) + jitter[i]) / res) mapped to  */
+/*   the roi, coord(c) = (c / (ry*rz), (c / rz) % ry, c % rz).               */
+/* sn_occ_ema_max: occs[c] = max(decay*occs[c], max_{i: idx[i]==c} occ[i]) - */
+/*   a deterministic scatter-max (the reference's index_put has undefined   */
+/*   order for duplicate indices, grid.py:232).                             */
+/* sn_occ_binarize: binary[c] = occs[c] > min(mean(occs), thre); the mean is */
+/*   an fp64 sum in fixed chunks of 1024 cells (sequential within a chunk,  */
+/*   chunks summed sequentially) so the bits are reproducible.  Also writes  */
+/*   the threshold used to *thre_out (device float, nullable).              */
+/* sn_occ_nonzero: ordered list of occupied cell indices + count.           */
+/* ------------------------------------------------------------------------ */
+int sn_occ_cell_points(const int64_t* cell_idx, int32_t n, const float* jitter,
+                       int32_t rx, int32_t ry, int32_t rz, const float* roi_aabb, float* x,
+                       void* stream);
+size_t sn_occ_workspace_bytes(int32_t n_cells);
+int sn_occ_ema_max(float* occs, int32_t n_cells, const int64_t* cell_idx, const float* occ,
+                   int32_t n, float decay, void* workspace, size_t workspace_bytes,
+                   void* stream);
+int sn_occ_binarize(const float* occs, int32_t n_cells, float thre, uint8_t* binary,
+                    float* thre_out, void* workspace, size_t workspace_bytes, void* stream);
+int sn_occ_nonzero(const uint8_t* binary, int32_t n_cells, int64_t* out_idx,
+                   int32_t* count, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Multiresolution hash-grid encoding: replaces tinycudann.Encoding with     */
+/* otype=HashGrid, 3-D input, linear interpolation (models/fields.py:25-31, */
+/* 78).  Level l: scale = base*2^(l*log2(per_level_scale)) - 1, res =       */
+/* ceil(scale)+1, pos = fma(scale, x, 0.5), cell = floor(pos) as uint32     */
+/* (inputs outside [0,1] wrap, as in tcnn), entry = dense index when res^3  */
+/* fits the level's table, else xor-prime hash (1, 2654435761, 805459861),  */
+/* modulo the level size.  params: float, level-major, features interleaved. */
+/* Output enc[i, l*F + f]; levels >= n_active_levels are written as 0 and   */
+/* receive no gradient (the bandwidth mask of fields.py:80-83).             */
+/* ------------------------------------------------------------------------ */
+/* Fills offsets[0..n_levels] (entries, not floats); returns total entries. */
+int64_t sn_hashgrid_layout(int32_t n_levels, int32_t log2_hashmap_size,
+                           int32_t base_resolution, float per_level_scale, uint32_t* offsets);
+int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_levels,
+                    int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
+                    float per_level_scale, int32_t n_active_levels, float* enc,
+                    void* stream);
+/* dL/dparams += sum over points (atomic fp32 adds; caller zeroes).  grad_x  */
+/* (nullable) = dL/dx (overwritten).                                         */
+int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
+                    int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
+                    float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                    float* grad_params, float* grad_x, void* stream);
+/* Backward of the map (grad_enc, params, x) -> grad_x = sum dL/denc * denc/dx */
+/* given its cotangent g_gx [n,3] (tcnn's backward_backward_input):         */
+/*   grad_grad_enc [n, L*F] (nullable, overwritten)                          */
+/*   grad_params  (nullable, atomically accumulated)                         */
+/*   grad_x [n,3] (nullable, overwritten): the trilinear cross terms         */
+int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
+                        int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
+                        float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                        const float* g_gx, float* grad_grad_enc, float* grad_params,
+                        float* grad_x, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
+/* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
+/* p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps).  `lr` is read from   */
+/* device memory (lr_dev) when non-null so that a captured graph can replay */
+/* with a schedule; else the scalar lr is used.                             */
+/* ------------------------------------------------------------------------ */
+int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                 int64_t n, float lr, const float* lr_dev, float beta1, float beta2,
+                 float eps, int32_t step, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SN_API_H */

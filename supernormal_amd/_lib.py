@@ -1,0 +1,113 @@
+"""ctypes binding of libsupernormal_amd.so (the C ABI declared in include/sn_api.h).
+
+The library is the only compute path of this package: if it is missing the
+import of any op fails with a RuntimeError telling how to build it.  There is
+no CPU or pure-PyTorch fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import torch
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("SN_LIB_PATH", _PKG / "_lib" / "libsupernormal_amd.so"))
+
+_i32, _i64, _f32, _sz = C.c_int32, C.c_int64, C.c_float, C.c_size_t
+_p = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "sn_last_error": (C.c_char_p, []),
+    "sn_abi_version": (C.c_int, []),
+    "sn_pack_info_workspace_bytes": (_sz, [_i32]),
+    "sn_pack_info": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _sz, _p]),
+    "sn_scan_workspace_bytes": (_sz, [_i64]),
+    "sn_counts_to_packed_info": (C.c_int, [_p, _i32, _p, _p, _p, _sz, _p]),
+    "sn_patch_weight_fwd": (C.c_int, [_p, _i32, _i32, _i64, _p, _p, _p, _p]),
+    "sn_patch_weight_bwd": (C.c_int, [_p, _i32, _i32, _i64, _p, _p, _p, _p, _p]),
+    "sn_patch_transmittance_fwd": (C.c_int, [_p, _i32, _i32, _i64, _p, _p, _p]),
+    "sn_visibility": (C.c_int, [_p, _i32, _i64, _p, _f32, _f32, _p, _p]),
+    "sn_compact_workspace_bytes": (_sz, [_i64]),
+    "sn_compact_samples": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "sn_ray_march_count": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
+                                     _f32, _i32, _p, _p]),
+    "sn_ray_march_fill": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
+                                    _f32, _i32, _p, _p, _p, _p, _p]),
+    "sn_scatter_add_rows": (C.c_int, [_p, _i64, _i32, _p, _p, _i32, _p]),
+    "sn_occ_cell_points": (C.c_int, [_p, _i32, _p, _i32, _i32, _i32, _p, _p, _p]),
+    "sn_occ_workspace_bytes": (_sz, [_i32]),
+    "sn_occ_ema_max": (C.c_int, [_p, _i32, _p, _p, _i32, _f32, _p, _sz, _p]),
+    "sn_occ_binarize": (C.c_int, [_p, _i32, _f32, _p, _p, _p, _sz, _p]),
+    "sn_occ_nonzero": (C.c_int, [_p, _i32, _p, _p, _p, _sz, _p]),
+    "sn_hashgrid_layout": (_i64, [_i32, _i32, _i32, _f32, _p]),
+    "sn_hashgrid_fwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p]),
+    "sn_hashgrid_bwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p,
+                                  _p, _p]),
+    "sn_hashgrid_bwd_bwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p,
+                                      _p, _p, _p, _p, _p]),
+    "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, _f32, _f32, _f32, _i32, _p]),
+}
+
+_lib = None
+
+
+class SNError(RuntimeError):
+    """A failed call into libsupernormal_amd (message from sn_last_error)."""
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"libsupernormal_amd.so not found at {LIB_PATH}; build it with "
+            "`python -m supernormal_amd.build` (requires hipcc / ROCm). "
+            "There is no fallback implementation.")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib() -> C.CDLL:
+    return load()
+
+
+def check(rc: int, name: str) -> None:
+    if rc != 0:
+        msg = load().sn_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{name}: {msg}")
+        raise SNError(f"{name} failed ({rc}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def ptr(t) -> int | None:
+    """Device address of a tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_of(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_cuda(*tensors, name: str = "op") -> None:
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise NotImplementedError(f"{name}: only GPU tensors are supported")
+
+
+def workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)

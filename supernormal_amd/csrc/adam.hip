@@ -1,0 +1,78 @@
+// Adam over one flat fp32 parameter buffer (torch.optim.Adam defaults used by
+// exp_runner.py:97, 205-207).  The buffer holds every trainable tensor of the
+// SDF (hash table, MLP, variance) back to back, so one launch updates all of
+// them; 28 bytes of HBM traffic per parameter (p, g, m, v read; p, m, v
+// written), float4-vectorised.
+#include <cmath>
+
+#include "sn_common.h"
+
+namespace sn {
+
+struct AdamK {
+  float b1, b2, eps, step_size, inv_sqrt_bc2;
+};
+
+__device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, const AdamK& k) {
+  m = k.b1 * m + (1.f - k.b1) * g;
+  v = k.b2 * v + (1.f - k.b2) * g * g;
+  const float denom = sqrtf(v) * k.inv_sqrt_bc2 + k.eps;
+  p = p - k.step_size * (m / denom);
+}
+
+__global__ void __launch_bounds__(256)
+adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+            float* __restrict__ v, int64_t n, AdamK k, float lr_scalar,
+            const float* __restrict__ lr_dev, float bc1) {
+  if (lr_dev) k.step_size = lr_dev[0] / bc1;
+  else k.step_size = lr_scalar / bc1;
+  const int64_t n4 = n / 4;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  const float4* g4 = reinterpret_cast<const float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
+    float4 pp = p4[i], gg = g4[i], mm = m4[i], vv = v4[i];
+    adam1(pp.x, gg.x, mm.x, vv.x, k);
+    adam1(pp.y, gg.y, mm.y, vv.y, k);
+    adam1(pp.z, gg.z, mm.z, vv.z, k);
+    adam1(pp.w, gg.w, mm.w, vv.w, k);
+    p4[i] = pp;
+    m4[i] = mm;
+    v4[i] = vv;
+  }
+  for (int64_t i = n4 * 4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    adam1(p[i], g[i], m[i], v[i], k);
+}
+
+}  // namespace sn
+
+using namespace sn;
+
+extern "C" {
+
+int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                 int64_t n, float lr, const float* lr_dev, float beta1, float beta2, float eps,
+                 int32_t step, void* stream) {
+  clear_error();
+  SN_REQUIRE(n >= 0 && step >= 1, "sn_adam_step: bad n=%lld / step=%d", (long long)n, step);
+  if (n == 0) return SN_OK;
+  SN_REQUIRE(params && grads && exp_avg && exp_avg_sq, "sn_adam_step: null pointer");
+  SN_REQUIRE(((uintptr_t)params | (uintptr_t)grads | (uintptr_t)exp_avg |
+              (uintptr_t)exp_avg_sq) % 16 == 0,
+             "sn_adam_step: buffers must be 16-byte aligned");
+  AdamK k;
+  k.b1 = beta1;
+  k.b2 = beta2;
+  k.eps = eps;
+  const double bc1 = 1.0 - std::pow((double)beta1, (double)step);
+  const double bc2 = 1.0 - std::pow((double)beta2, (double)step);
+  k.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
+  k.step_size = 0.f;
+  adam_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, as_stream(stream)>>>(
+      params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1);
+  return check_launch("sn_adam_step");
+}
+
+}  // extern "C"
