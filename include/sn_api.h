@@ -201,6 +201,42 @@ int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t 
                         float* grad_x, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* SDF MLP (models/fields.py:39-99 with input_concat): h0 = [x, enc] of     */
+/* width d_in = 3 + enc_dims (<= 64); n_layers hidden layers (1 or 2) of     */
+/* width d_hidden (32 or 64) with Softplus(beta), threshold 20; output 1.    */
+/* Weights are the weight-normed (folded) matrices, packed fp32 row-major:   */
+/*   W0[H][d_in] b0[H] { W1[H][H] b1[H] } wo[H] bo                           */
+/* Runs on the fp32 matrix cores.  n_cap is the buffer capacity (rows);      */
+/* n_dev (nullable, device int32) the number of valid rows, so callers can   */
+/* size work on the device without a host round trip.                        */
+/* ------------------------------------------------------------------------ */
+int sn_mlp_n_params(int32_t d_in, int32_t d_hidden, int32_t n_layers);
+/* sdf[n]; dsdf_dh0 (nullable) [n, d_in] = d sdf / d h0 (for the analytic    */
+/* x-gradient, fields.py:107-119: grad = dsdf_dh0[:, :3] + the hash-grid     */
+/* backward of dsdf_dh0[:, 3:]).                                             */
+int sn_mlp_fwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+               const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+               float beta, float* sdf, float* dsdf_dh0, void* stream);
+size_t sn_mlp_bwd_workspace_bytes(int64_t n_cap, int32_t enc_dims, int32_t d_hidden,
+                                  int32_t n_layers, int32_t analytic);
+/* Backward of sum g_sdf * sdf: g_mlp (packed layout; overwritten, or added  */
+/* to when accumulate != 0) summed over rows in a fixed order; g_enc [n,E]   */
+/* and g_x [n,3] (both nullable) overwritten.                                */
+int sn_mlp_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+               const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+               float beta, const float* g_sdf, float* g_enc, float* g_x, float* g_mlp,
+               int32_t accumulate, void* workspace, size_t workspace_bytes, void* stream);
+/* Backward of J = sum g_sdf * sdf + v . (d sdf/dx) (double backward of the   */
+/* analytic gradient).  enc_dot [n,E] = directional derivative of enc along */
+/* v (sn_hashgrid_bwd_bwd's grad_grad_enc with g_gx = v).  Outputs g_mlp as  */
+/* sn_mlp_bwd, g_h0 = dJ/dh0 and g_h0_dot = dJ/d(h0 tangent), both [n, d_in]. */
+int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float* enc_dot,
+                  int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                  int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                  float* g_h0, float* g_h0_dot, float* g_mlp, int32_t accumulate,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
 /* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
 /* p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps).  `lr` is read from   */

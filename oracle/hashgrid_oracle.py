@@ -50,17 +50,21 @@ def hashgrid_desc(n_levels, log2_hashmap_size, base_resolution, per_level_scale)
     }
 
 
-def encode(x, params, desc, n_features=2, n_active=None):
-    """enc[i, l*F+f] for x [n,3] and flat params (fp64 tensors; autograd-able)."""
+def encode(x, params, desc, n_features=2, n_active=None, fp32_pos=True):
+    """enc[i, l*F+f] for x [n,3] and flat params (fp64 tensors; autograd-able).
+
+    fp32_pos=True (the method as specified): fractional positions are the fp32
+    values of fmaf(scale, x, 0.5) - cell.  fp32_pos=False: the same corners with
+    fp64 positions, a smooth model for finite-difference checks."""
     L = desc["n_levels"]
     F = n_features
     if n_active is None:
         n_active = L
     n = x.shape[0]
     xnp = x.detach().to(torch.float32).cpu().numpy()
-    idx, _frac32, cell = c_oracle.hashgrid_corners(xnp, desc)
+    idx, frac32, cell32 = c_oracle.hashgrid_corners(xnp, desc)
     idx_t = torch.from_numpy(idx.astype(np.int64))
-    cell_t = torch.from_numpy(cell).to(x.dtype)
+    frac32_t = torch.from_numpy(frac32).to(x.dtype)
     table = params.view(-1, F)
     outs = []
     for l in range(L):
@@ -68,7 +72,13 @@ def encode(x, params, desc, n_features=2, n_active=None):
             outs.append(torch.zeros(n, F, dtype=x.dtype))
             continue
         sc = float(desc["scale"][l])
-        frac = sc * x + 0.5 - cell_t[:, l]  # [n,3], differentiable
+        # The position is part of the method's definition in fp32 (fmaf(scale, x, 0.5)
+        # rounded to fp32, ulp ~3e-5 at scale ~270): the value is the fp32 fraction,
+        # the derivative d frac / dx = scale is exact.
+        if fp32_pos:
+            frac = frac32_t[:, l] + (sc * x - (sc * x).detach())  # [n,3]
+        else:
+            frac = sc * x + 0.5 - torch.from_numpy(cell32[:, l]).to(x.dtype)
         acc = 0.0
         for c in range(8):
             w = torch.ones(n, dtype=x.dtype)

@@ -48,6 +48,13 @@ SIGNATURES = {
                                   _p, _p]),
     "sn_hashgrid_bwd_bwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p,
                                       _p, _p, _p, _p, _p]),
+    "sn_mlp_n_params": (C.c_int, [_i32, _i32, _i32]),
+    "sn_mlp_bwd_workspace_bytes": (_sz, [_i64, _i32, _i32, _i32, _i32]),
+    "sn_mlp_fwd": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p, _p]),
+    "sn_mlp_bwd": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p, _p, _p,
+                             _i32, _p, _sz, _p]),
+    "sn_mlp_bwd_ad": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p,
+                                _p, _p, _i32, _p, _sz, _p]),
     "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, _f32, _f32, _f32, _i32, _p]),
 }
 

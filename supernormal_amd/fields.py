@@ -124,6 +124,38 @@ class SDFNetwork(nn.Module):
                                         only_inputs=True)[0]
         return gradients.unsqueeze(1)
 
+    # -- fused HIP evaluation ------------------------------------------------
+    def sdf_config(self):
+        """Shape of this field for the fused kernels (raises if unsupported)."""
+        from .sdf import SDFConfig
+        n_hidden = self.num_layers - 2
+        ok = (self.encoding is not None and self.input_concat and self.d_in == 3
+              and self.dims[-1] == 1 and all(l not in self.skip_in for l in range(self.num_layers))
+              and n_hidden in (1, 2) and all(h == self.dims[1] for h in self.dims[1:-1])
+              and self.dims[1] in (32, 64)
+              and (self.bindwidth * 2) % self.encoding.grid["n_features"] == 0)
+        if not ok:
+            raise ValueError("fused SDF kernels need input_concat, no skips, d_out 1, 1-2 hidden "
+                             "layers of width 32 or 64 and 2*bandwidth divisible by the "
+                             "features per level")
+        return SDFConfig(self.encoding.grid, self.dims[1], n_hidden, beta=100.0)
+
+    def packed_mlp(self):
+        from .sdf import pack_mlp
+        return pack_mlp(self.mlp_layers())
+
+    def fused_sdf(self, x):
+        """sdf [N] via hash encode + MFMA MLP kernels (same values as sdf(x)[:, 0])."""
+        from .sdf import sdf
+        return sdf(x, self.encoding.params, self.packed_mlp(), self.sdf_config(),
+                   self.active_levels())
+
+    def fused_sdf_and_gradient(self, x):
+        """(sdf [N], d sdf / dx [N,3]), twice differentiable w.r.t. the parameters."""
+        from .sdf import sdf_and_gradient
+        return sdf_and_gradient(x, self.encoding.params, self.packed_mlp(), self.sdf_config(),
+                                self.active_levels())
+
     # -- folded MLP weights (used by the fused kernels) ----------------------
     def mlp_layers(self):
         """[(W [out,in], b [out])] with weight norm applied (differentiable)."""

@@ -289,15 +289,18 @@ def test_occ_kernels_vs_oracle(dev):
     jit = rng.random((50000, 3), dtype=np.float32)
     roi = np.array([-1, -1, -1, 1, 1, 1], np.float32)
     x = torch.empty(50000, 3, device=dev)
-    call("sn_occ_cell_points", ptr(_t(idx, dev)), 50000, ptr(_t(jit, dev)), *res, ptr(_t(roi, dev)),
-         ptr(x), stream_of(dev))
+    # device copies are held in locals: a raw pointer does not keep a tensor alive
+    idx_d, jit_d, roi_d = _t(idx, dev), _t(jit, dev), _t(roi, dev)
+    call("sn_occ_cell_points", ptr(idx_d), 50000, ptr(jit_d), *res, ptr(roi_d), ptr(x),
+         stream_of(dev))
     np.testing.assert_array_equal(x.cpu().numpy(), c_oracle.occ_cell_points(idx, jit, res, roi))
     occs0 = rng.random(n_cells, dtype=np.float32)
     occ = rng.random(50000, dtype=np.float32)
     occs = _t(occs0, dev)
     ws = workspace(_lib.lib().sn_occ_workspace_bytes(n_cells), dev)
-    call("sn_occ_ema_max", ptr(occs), n_cells, ptr(_t(idx, dev)), ptr(_t(occ, dev)), 50000,
-         0.95, ptr(ws), ws.numel(), stream_of(dev))
+    occ_d = _t(occ, dev)
+    call("sn_occ_ema_max", ptr(occs), n_cells, ptr(idx_d), ptr(occ_d), 50000, 0.95, ptr(ws),
+         ws.numel(), stream_of(dev))
     want = c_oracle.occ_ema_max(occs0, idx, occ, 0.95)
     np.testing.assert_array_equal(occs.cpu().numpy(), want)
     binary = torch.empty(n_cells, dtype=torch.uint8, device=dev)

@@ -282,7 +282,10 @@ def test_hashgrid_dense_level_reproduces_affine():
     # inputs inside [0, 1) so no corner wraps: cells stay within [0, res-1)
     x = torch.rand(300, 3, dtype=torch.float64) * (res - 1.5) / sc
     enc = encode(x, params, d, n_active=1)
-    pos = sc * x + 0.5
+    # pos = fmaf(scale, x, 0.5) in fp32 (= cell + frac from the corner selection)
+    _, frac, cell = c_oracle.hashgrid_corners(x.numpy().astype(np.float32), d)
+    pos = torch.from_numpy(cell[:, 0].astype(np.float64) + frac[:, 0].astype(np.float64))
+    torch.testing.assert_close(pos, sc * x.float().double() + 0.5, rtol=0, atol=1e-5)
     want = pos @ torch.tensor(A) + 0.1
     torch.testing.assert_close(enc[:, 0], want, rtol=1e-9, atol=1e-9)
 
@@ -316,7 +319,7 @@ def test_hashgrid_grad_fd():
     p = params.clone().requires_grad_(True)
 
     def f(xx, pp):
-        return encode(xx, pp, d)
+        return encode(xx, pp, d, fp32_pos=False)
 
     # finite differences stay inside one cell for tiny eps
     assert torch.autograd.gradcheck(f, (x, p), eps=1e-7, atol=1e-5, rtol=1e-4)
