@@ -237,6 +237,35 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
                   void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Fused patch rendering + losses, forward and backward: replaces the torch  */
+/* graph of models/renderer.py:142-276 (NeuS alpha, patch weights, 'dfd' or  */
+/* 'ad' per-sample normals, accumulation) and exp_runner.py:169-203 (normal  */
+/* l2/l1, eikonal, mask BCE) and its autograd backward.                      */
+/* Per patch b (n_patches, patch_h x patch_w rays, centre ray = middle):     */
+/*   rays_o, rays_d, n_true [B,P,3], plane_n [B,3], v_inv [B,P,3,3] ('dfd'),  */
+/*   mask [B,P] (binarised), packed_info [B,2] of the centre samples,        */
+/*   t_starts/t_ends [S] (centre ray), diff_idx [S] = row of the sample's end */
+/*   in sdf_ends or -1 when the end is the next sample's start (same patch), */
+/*   sdf_starts [S,P], sdf_ends [Nd,P], grad_starts [S,P,3] ('ad', else null),*/
+/*   inv_s, mask_sum (= sum(mask) + 1e-5) and n_samples (= S) on the device.  */
+/* Outputs: dL/dsdf_starts [S,P], dL/dsdf_ends [Nd,P], dL/dgrad [S,P,3]      */
+/* ('ad'), losses[4] = (normal, mask, eikonal, dL/dinv_s) (weighted terms),  */
+/* weight_sum [B,P] and comp_normal [B,P,3] (nullable).  partials needs      */
+/* 4 * sn_render_step_waves() floats.                                        */
+/* ------------------------------------------------------------------------ */
+int sn_render_step_waves(int32_t n_patches, int32_t patch_size);
+int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const float* rays_o,
+                   const float* rays_d, const float* plane_n, const float* v_inv,
+                   const float* n_true, const float* mask, const int32_t* packed_info,
+                   const float* t_starts, const float* t_ends, const int32_t* diff_idx,
+                   const float* sdf_starts, const float* sdf_ends, const float* grad_starts,
+                   const float* inv_s, const float* mask_sum, const int32_t* n_samples,
+                   float normal_w, float eik_w, float mask_w, int32_t loss_l1,
+                   float* g_sdf_starts, float* g_sdf_ends, float* g_grad_starts,
+                   float* losses, float* partials, float* weight_sum, float* comp_normal,
+                   void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
 /* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
 /* p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps).  `lr` is read from   */

@@ -55,6 +55,9 @@ SIGNATURES = {
                              _i32, _p, _sz, _p]),
     "sn_mlp_bwd_ad": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p,
                                 _p, _p, _i32, _p, _sz, _p]),
+    "sn_render_step_waves": (C.c_int, [_i32, _i32]),
+    "sn_render_step": (C.c_int, [_i32, _i32, _i32] + [_p] * 16 + [_f32, _f32, _f32, _i32]
+                       + [_p] * 8),
     "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, _f32, _f32, _f32, _i32, _p]),
 }
 

@@ -1,0 +1,212 @@
+"""Evaluation of a trained field: visible surface points and Chamfer / F-score.
+
+The reference scores a run by extracting a marching-cubes mesh
+(renderer.py:9-34), intersecting it with the rays of the training pixels
+inside the masks (exp_runner.py:618-630, first hits only) and comparing the
+hits with the same points on the ground-truth mesh by Chamfer distance and
+F-score (models/cd_and_fscore.py:5-29).
+
+Here the first hits are taken on the learned SDF directly: each pixel ray is
+marched from the unit sphere with a fixed step until the SDF changes sign,
+and the crossing is refined by bisection.  That is the point the reference's
+ray-mesh intersection converges to as the marching-cubes resolution grows,
+without building a mesh.  Chamfer / F-score use exact nearest neighbours
+computed on the GPU in fp64 (brute force in tiles), i.e. the same numbers
+scipy's KDTree gives in cd_and_fscore.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+@torch.no_grad()
+def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=1 << 16):
+    """First + to - crossing of sdf_fn along each ray in [near, far]; returns
+    (points [M,3], hit mask [N])."""
+    dev = rays_o.device
+    N = rays_o.shape[0]
+    pts = torch.zeros(N, 3, device=dev)
+    hit = torch.zeros(N, dtype=torch.bool, device=dev)
+    for s in range(0, N, chunk):
+        o, d = rays_o[s:s + chunk], rays_d[s:s + chunk]
+        t_lo = near[s:s + chunk].clone()
+        t_hi_lim = far[s:s + chunk]
+        f_lo = sdf_fn(o + d * t_lo[:, None])
+        done = torch.zeros(o.shape[0], dtype=torch.bool, device=dev)
+        found = torch.zeros_like(done)
+        t_hit_lo = torch.zeros_like(t_lo)
+        t_hit_hi = torch.zeros_like(t_lo)
+        t = t_lo.clone()
+        while not bool(done.all()):
+            act = ~done
+            t_new = torch.where(act, t + step, t)
+            f_new = torch.full_like(f_lo, 1.0)
+            idx = act.nonzero()[:, 0]
+            f_new[idx] = sdf_fn(o[idx] + d[idx] * t_new[idx, None])
+            cross = act & (f_lo > 0) & (f_new <= 0)
+            t_hit_lo = torch.where(cross, t, t_hit_lo)
+            t_hit_hi = torch.where(cross, t_new, t_hit_hi)
+            found |= cross
+            done |= cross | (t_new >= t_hi_lim)
+            t = t_new
+            f_lo = torch.where(act, f_new, f_lo)
+        a, b = t_hit_lo, t_hit_hi
+        for _ in range(n_bisect):
+            m = 0.5 * (a + b)
+            fm = sdf_fn(o + d * m[:, None])
+            a = torch.where(fm > 0, m, a)
+            b = torch.where(fm > 0, b, m)
+        tm = 0.5 * (a + b)
+        pts[s:s + chunk] = o + d * tm[:, None]
+        hit[s:s + chunk] = found
+    return pts[hit], hit
+
+
+@torch.no_grad()
+def visible_surface_points(sdf_fn, dataset, rays_per_view=20000, seed=0, step=2e-3):
+    """First hits of the learned SDF along masked training-pixel rays."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    out = []
+    for i in dataset.train_images:
+        m = dataset.masks[i].reshape(-1).nonzero()[:, 0].cpu()
+        if m.numel() == 0:
+            continue
+        pick = m[torch.randint(0, m.numel(), (min(rays_per_view, m.numel()),), generator=g)]
+        ys, xs = (pick // dataset.W).to(dataset.device), (pick % dataset.W).to(dataset.device)
+        p = torch.stack([xs.float(), ys.float(), torch.ones_like(xs, dtype=torch.float32)], -1)
+        p = p @ dataset.intrinsics_all_inv[i, :3, :3].T
+        d = p / p.norm(dim=-1, keepdim=True)
+        d = d @ dataset.pose_all[i, :3, :3].T
+        o = dataset.pose_all[i, :3, 3].expand(d.shape).contiguous()
+        near, far = dataset.near_far_from_sphere(o, d)
+        pts, _ = first_hits(sdf_fn, o, d, near, far, step=step)
+        out.append(pts)
+    return torch.cat(out).cpu().numpy()
+
+
+@torch.no_grad()
+def nn_distances(a, b, tile=4096):
+    """For each point of a, the fp64 Euclidean distance to its nearest point of b."""
+    a = torch.as_tensor(a, dtype=torch.float64)
+    b = torch.as_tensor(b, dtype=torch.float64)
+    dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+    a, b = a.to(dev), b.to(dev)
+    out = torch.empty(a.shape[0], dtype=torch.float64, device=dev)
+    bb = (b * b).sum(-1)
+    for s in range(0, a.shape[0], tile):
+        x = a[s:s + tile]
+        best = torch.full((x.shape[0],), float("inf"), dtype=torch.float64, device=dev)
+        xx = (x * x).sum(-1, keepdim=True)
+        for t in range(0, b.shape[0], 1 << 15):
+            y = b[t:t + (1 << 15)]
+            d2 = xx + bb[t:t + (1 << 15)][None, :] - 2.0 * (x @ y.T)
+            best = torch.minimum(best, d2.min(dim=1).values)
+        out[s:s + tile] = best.clamp_min(0).sqrt()
+    return out.cpu().numpy()
+
+
+def chamfer_distance_and_f1_score(ref_points, eval_points, f_threshold=0.5):
+    """models/cd_and_fscore.py:5-29 with exact fp64 nearest neighbours:
+    CD = (mean d(eval->ref) + mean d(ref->eval)) / 2, F = 2PR/(P+R) at f_threshold."""
+    d_e2r = nn_distances(eval_points, ref_points)
+    d_r2e = nn_distances(ref_points, eval_points)
+    cd = (np.mean(d_e2r) + np.mean(d_r2e)) / 2
+    precision = np.mean(d_e2r < f_threshold)
+    recall = np.mean(d_r2e < f_threshold)
+    f = 2 * precision * recall / (precision + recall) if precision + recall > 0 else 0.0
+    return float(cd), float(f)
+
+
+def extract_fields(bound_min, bound_max, resolution, query_func, chunk=64, device=None):
+    """SDF values on a resolution^3 grid (renderer.py:9-24)."""
+    device = device or (torch.device("cuda") if torch.cuda.is_available() else "cpu")
+    axes = [torch.linspace(float(bound_min[i]), float(bound_max[i]), resolution) for i in range(3)]
+    u = np.zeros([resolution] * 3, dtype=np.float32)
+    with torch.no_grad():
+        for xi, xs in enumerate(axes[0].split(chunk)):
+            for yi, ys in enumerate(axes[1].split(chunk)):
+                for zi, zs in enumerate(axes[2].split(chunk)):
+                    xx, yy, zz = torch.meshgrid(xs, ys, zs, indexing="ij")
+                    pts = torch.stack([xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)], -1)
+                    val = query_func(pts.to(device)).reshape(len(xs), len(ys), len(zs))
+                    u[xi * chunk: xi * chunk + len(xs), yi * chunk: yi * chunk + len(ys),
+                      zi * chunk: zi * chunk + len(zs)] = val.detach().cpu().numpy()
+    return u
+
+
+# Marching tetrahedra: each grid cube is split into 6 tetrahedra along the
+# main diagonal; each tetrahedron yields 0, 1 or 2 triangles.  (The
+# reference uses PyMCubes' marching cubes, which is not installed; the
+# surfaces agree up to the triangulation inside each cube.)
+_CUBE_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                          [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+_TETS = np.array([[0, 5, 1, 6], [0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6],
+                  [0, 4, 5, 6]])
+
+
+def marching_tetrahedra(u, threshold=0.0):
+    """Triangle mesh of {u = threshold}: (vertices [V,3] in grid units, faces [F,3])."""
+    R = np.array(u.shape)
+    idx = np.stack(np.meshgrid(*[np.arange(r - 1) for r in R], indexing="ij"), -1).reshape(-1, 3)
+    vals = np.stack([u[idx[:, 0] + c[0], idx[:, 1] + c[1], idx[:, 2] + c[2]] for c in _CUBE_CORNERS],
+                    -1) - threshold
+    inside = vals < 0
+    any_cross = inside.any(1) & (~inside).any(1)
+    idx, vals = idx[any_cross], vals[any_cross]
+    tris = []
+    for tet in _TETS:
+        tv = vals[:, tet]
+        tp = (idx[:, None, :] + _CUBE_CORNERS[tet][None]).astype(np.float64)
+        neg = tv < 0
+        code = neg.astype(np.int64) @ np.array([1, 2, 4, 8])
+        # direction from the inside corners to the outside corners (orientation)
+        n_in = neg.sum(1, keepdims=True)
+        out_dir = (tp * (~neg)[..., None]).sum(1) / np.maximum(4 - n_in, 1) - \
+            (tp * neg[..., None]).sum(1) / np.maximum(n_in, 1)
+
+        def edge(i, j, sel):
+            a, b = tv[sel, i], tv[sel, j]
+            t = a / (a - b)
+            return tp[sel, i] + (tp[sel, j] - tp[sel, i]) * t[:, None]
+
+        for lone in range(4):  # one corner on its own side: one triangle
+            others = [k for k in range(4) if k != lone]
+            for c in (1 << lone, 15 - (1 << lone)):
+                sel = code == c
+                if sel.any():
+                    tri = np.stack([edge(lone, k, sel) for k in others], 1)
+                    tris.append((tri, out_dir[sel]))
+        for a_, b_ in ((0, 1), (0, 2), (0, 3)):  # two and two: a quad
+            c_ = [k for k in range(4) if k not in (a_, b_)]
+            for c in ((1 << a_) | (1 << b_), 15 - ((1 << a_) | (1 << b_))):
+                sel = code == c
+                if sel.any():
+                    p0, p1 = edge(a_, c_[0], sel), edge(a_, c_[1], sel)
+                    p2, p3 = edge(b_, c_[1], sel), edge(b_, c_[0], sel)
+                    tris.append((np.stack([p0, p1, p2], 1), out_dir[sel]))
+                    tris.append((np.stack([p0, p2, p3], 1), out_dir[sel]))
+    verts, faces = [], []
+    nv = 0
+    for tri, od in tris:
+        nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+        flip = (nrm * od).sum(1) < 0
+        tri = np.where(flip[:, None, None], tri[:, ::-1], tri)
+        n = tri.shape[0]
+        verts.append(tri.reshape(-1, 3))
+        faces.append(nv + np.arange(3 * n).reshape(n, 3))
+        nv += 3 * n
+    if not verts:
+        return np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64)
+    return np.concatenate(verts), np.concatenate(faces)
+
+
+def extract_geometry(bound_min, bound_max, resolution, threshold, query_func):
+    """(vertices in world units, triangles) of {query_func = threshold}
+    (renderer.py:27-34; query_func is -sdf there)."""
+    u = extract_fields(bound_min, bound_max, resolution, query_func)
+    v, f = marching_tetrahedra(-u, -threshold)
+    bmin = np.asarray(bound_min, dtype=np.float64)
+    bmax = np.asarray(bound_max, dtype=np.float64)
+    v = v / (resolution - 1.0) * (bmax - bmin)[None, :] + bmin[None, :]
+    return v, f
