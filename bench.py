@@ -1,0 +1,204 @@
+"""Training throughput of SuperNormal's hot path on one MI355X.
+
+Workload (the full-size single-object configuration): seeded smooth union of
+a torus (R 0.4, r 0.12), a 0.3-side box and 3 spheres; 20 ring + elevation
+cameras at 612x512 with N(0, 0.02) normal noise + masks; hash grid 16 levels
+x 2 features, T = 2^19, base 16, x1.38; SDF MLP 64 wide, 2 hidden layers;
+2048 patches of 3x3 (18 432 rays) per iteration; occupancy grid 128^3; <= 128
+samples per ray; 5000 iterations with the reference's schedules.  A "step" is
+one training iteration (occupancy update when due, patch sampling, marching,
+rendering, losses, backward, Adam) on the fused HIP path.
+
+Timed region: W untimed warm-up iterations, then K iterations bracketed by
+device synchronisation; the default W + K = 5000 covers the whole schedule,
+so `value` is the run-average iterations/s.  Beside it:
+  * plain_gpu: the reference-structured path (torch autograd over the drop-in
+    kernels, torch Linear MLP) timed from the same trained state;
+  * roofline: the dominant kernel's algorithmic bytes per launch / its mean
+    duration (HIP events on the launch stream over the timed region);
+  * cpu_baseline: the CPU oracle (torch restatement of the reference
+    iteration) on the same workload, a bounded number of iterations;
+  * accuracy: Chamfer distance / F-score of the visible surface points of the
+    trained field against the analytic scene's.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hash_bwd_bytes(args, grid):
+    """Algorithmic bytes of one sn_hashgrid_bwd launch: positions (12 B) and
+    dL/denc (L*F*4 B) read per point, plus an fp32 read-modify-write of the 8
+    corners x F features of every active level."""
+    n = int(args[1])
+    n_active = int(args[8])
+    L, F = grid["n_levels"], grid["n_features"]
+    return n * (12 + L * F * 4 + n_active * 8 * F * 4 * 2)
+
+
+def hash_fwd_bytes(args, grid):
+    """Positions read, 8 corners x F fp32 gathered per active level, L*F written."""
+    n = int(args[1])
+    n_active = int(args[8])
+    L, F = grid["n_levels"], grid["n_features"]
+    return n * (12 + n_active * 8 * F * 4 + L * F * 4)
+
+
+ROOFLINE_KERNELS = {"sn_hashgrid_bwd": hash_bwd_bytes, "sn_hashgrid_fwd": hash_fwd_bytes}
+
+
+def cpu_baseline(runner, iters=2):
+    """The oracle's CPU iteration on the same workload, bounded to `iters`
+    iterations after one untimed iteration."""
+    from oracle.train_oracle import OracleTrainer
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    ds = runner.dataset
+    tr = OracleTrainer(runner.sdf_network, float(runner.deviation_network.variance),
+                       runner.occ_resolution, {"lr": runner.learning_rate, "loss_type": "l2"})
+    tr.binary = runner.renderer.occupancy_grid.binary.cpu().numpy().astype(np.uint8)
+    tr.occs = runner.renderer.occupancy_grid.occs.cpu().numpy()
+    g = torch.Generator(device=ds.device).manual_seed(123)
+    times = []
+    step = float(runner.renderer.sampling_step_size)
+    for i in range(iters + 1):
+        o, d, pn, vi, nt, m = ds.gen_random_patches(runner.batch_size, generator=g)
+        near, far = ds.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+        cpu = [t.cpu() for t in (o, d, pn, vi, nt, m, near, far)]
+        t0 = time.perf_counter()
+        tr.step(*cpu, step_size=step, lr=runner.lr, cap=runner.max_samples_per_ray,
+                method=runner.gradient_method)
+        if i > 0:
+            times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    return {"value": 1.0 / sec, "unit": "train iters/s", "cores": threads, "kind": "port",
+            "sample": f"{iters} iterations of the CPU oracle (torch fp32 restatement of the "
+                      f"reference iteration) at 2048 patches from the trained state, median"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4980)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="tbs612")
+    ap.add_argument("--method", default="dfd", choices=["dfd", "ad"])
+    ap.add_argument("--roofline-kernel", default="sn_hashgrid_bwd")
+    ap.add_argument("--plain-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
+        return 2
+    from supernormal_amd import _lib
+    from supernormal_amd.runner import Runner, synthetic_conf
+
+    dev = torch.device("cuda:0")
+    conf = synthetic_conf(a.workload, gradient_method=a.method,
+                          end_iter=max(a.steps + a.warmup, 1))
+    runner = Runner(conf, fused=True, device=dev)
+    grid = runner.sdf_network.encoding.grid
+    runner.update_learning_rate()
+    torch.cuda.synchronize()
+    for i in range(a.warmup):
+        runner.train_iteration(i)
+    torch.cuda.synchronize()
+    timer = _lib.time_calls(a.roofline_kernel)
+    samples = 0
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(a.warmup, a.warmup + a.steps):
+        runner.train_iteration(i)
+        samples += getattr(runner, "last_samples", 0)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    _lib.stop_timing(a.roofline_kernel)
+    ms_per_step = ms / max(a.steps, 1)
+    value = 1000.0 / ms_per_step
+    durs = timer.durations_ms()
+    byte_fn = ROOFLINE_KERNELS[a.roofline_kernel]
+    nbytes = [byte_fn(args, grid) for _, _, args in timer.records]
+    roof = None
+    if durs:
+        avg_ms = float(np.mean(durs))
+        avg_bytes = float(np.mean(nbytes))
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": a.roofline_kernel, "launches": len(durs),
+                "avg_launch_ms": round(avg_ms, 4), "avg_algorithmic_bytes": int(avg_bytes)}
+    out = {
+        "metric": "train iters/s at 18k rays vs plain GPU @ equal Chamfer; p50/p99 job time-to-mesh",
+        "value": round(value, 2), "unit": "train iters/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: torus+box+3 spheres, 20 views 612x512, hash 16x2 "
+                               f"T=2^19 base 16 x1.38, MLP 64x2, 2048 3x3 patches (18432 rays), "
+                               f"occupancy 128^3, <=128 samples/ray, gradient {a.method}",
+                   "global_batch": runner.batch_size * 9, "iterations": a.warmup + a.steps,
+                   "parallelism": "single GPU"},
+        "samples_per_iter": round(samples / max(a.steps, 1), 1),
+        "wall_s": round(wall, 2),
+        "roofline": roof,
+    }
+    if not a.no_accuracy:
+        from supernormal_amd.mesh import chamfer_distance_and_f1_score, visible_surface_points
+        net = runner.sdf_network
+        pts = visible_surface_points(lambda x: net.fused_sdf(x), runner.dataset, 4000)
+        gt = runner.dataset.surface_points(4000 * len(runner.dataset.train_images))
+        cd, fs = chamfer_distance_and_f1_score(gt, pts, f_threshold=0.005)
+        out["accuracy"] = {"chamfer": round(cd, 6), "fscore@0.005": round(fs, 4),
+                           "n_eval_points": int(len(pts))}
+    if not a.no_plain:
+        runner.fused = False
+        torch.cuda.synchronize()
+        it0 = runner.iter_step
+        runner.train_iteration(it0)  # untimed
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(a.plain_iters):
+            runner.train_iteration(it0 + 1 + i)
+        torch.cuda.synchronize()
+        plain = a.plain_iters / (time.perf_counter() - t)
+        runner.fused = True
+        t = time.perf_counter()
+        for i in range(a.plain_iters):
+            runner.train_iteration(it0 + 1 + a.plain_iters + i)
+        torch.cuda.synchronize()
+        fused_same = a.plain_iters / (time.perf_counter() - t)
+        out["plain_gpu"] = {"value": round(plain, 2), "unit": "train iters/s",
+                            "path": "reference-structured (autograd over drop-in kernels, "
+                                    "torch Linear MLP)",
+                            "fused_same_state": round(fused_same, 2),
+                            "speedup": round(fused_same / plain, 2)}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(runner, a.cpu_iters)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -98,8 +98,44 @@ def check(rc: int, name: str) -> None:
         raise SNError(f"{name} failed ({rc}): {msg}")
 
 
+class LaunchTimer:
+    """Brackets every call of one entry point with HIP events on the current
+    stream (the stream the kernels are enqueued on) and records the argument
+    tuple, so a caller can compute per-launch bytes / duration."""
+
+    def __init__(self, name: str):
+        self.name = name
+        self.records = []  # (start_event, end_event, args)
+
+    def durations_ms(self):
+        torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b, _ in self.records]
+
+
+_timers = {}
+
+
+def time_calls(name: str) -> LaunchTimer:
+    t = LaunchTimer(name)
+    _timers[name] = t
+    return t
+
+
+def stop_timing(name: str) -> None:
+    _timers.pop(name, None)
+
+
 def call(name: str, *args) -> None:
+    t = _timers.get(name)
+    if t is None:
+        check(getattr(load(), name)(*args), name)
+        return
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     check(getattr(load(), name)(*args), name)
+    e1.record()
+    t.records.append((e0, e1, args))
 
 
 def ptr(t) -> int | None:
