@@ -118,18 +118,19 @@ int sn_compact_samples(const uint8_t* keep, int32_t n, const int64_t* ray_indice
 /* that many samples (0 = unbounded, the reference's behaviour).  Floating  */
 /* point follows the reference kernel with no FMA contraction.              */
 /* grid: uint8 [rx, ry, rz] (x-major, index x*ry*rz + y*rz + z).            */
+/* step_dev (nullable): device float overriding step_size (no host sync).   */
 /* ------------------------------------------------------------------------ */
 int sn_ray_march_count(const float* rays_o, const float* rays_d, const float* t_min,
                        const float* t_max, int32_t n_rays, const float* roi_aabb,
                        const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
-                       float step_size, float cone_angle, int32_t max_per_ray,
-                       int32_t* num_steps, void* stream);
+                       float step_size, const float* step_dev, float cone_angle,
+                       int32_t max_per_ray, int32_t* num_steps, void* stream);
 int sn_ray_march_fill(const float* rays_o, const float* rays_d, const float* t_min,
                       const float* t_max, int32_t n_rays, const float* roi_aabb,
                       const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
-                      float step_size, float cone_angle, int32_t max_per_ray,
-                      const int32_t* packed_info, int64_t* ray_indices, float* t_starts,
-                      float* t_ends, void* stream);
+                      float step_size, const float* step_dev, float cone_angle,
+                      int32_t max_per_ray, const int32_t* packed_info, int64_t* ray_indices,
+                      float* t_starts, float* t_ends, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Patch accumulation: replaces accumulate_along_rays_patch_based            */
@@ -184,7 +185,10 @@ int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_le
                     float per_level_scale, int32_t n_active_levels, float* enc,
                     void* stream);
 /* dL/dparams += sum over points (atomic fp32 adds; caller zeroes).  grad_x  */
-/* (nullable) = dL/dx (overwritten).                                         */
+/* (nullable) = dL/dx (overwritten).  Coarse levels (scale < 192) first sum */
+/* the lanes of a wavefront that hit the same table row, unless disabled by */
+/* sn_hashgrid_set_aggregation(0) (process-wide switch, for measurement).   */
+void sn_hashgrid_set_aggregation(int32_t on);
 int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
                     int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
                     float per_level_scale, int32_t n_active_levels, const float* grad_enc,
@@ -273,8 +277,62 @@ int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const fl
 /* with a schedule; else the scalar lr is used.                             */
 /* ------------------------------------------------------------------------ */
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
-                 int64_t n, float lr, const float* lr_dev, float beta1, float beta2,
-                 float eps, int32_t step, void* stream);
+                 int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
+                 double eps, int32_t step, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Device-count variants for an iteration with no host round trip.  Each    */
+/* takes a capacity (`n_cap`, sizes the launch and the buffers) and a device */
+/* int32 count `n_dev` (the valid prefix; nullable = n_cap), so sizes that  */
+/* depend on the data never travel to the host.  Same results as the plain  */
+/* entry points on the valid prefix.                                        */
+/* ------------------------------------------------------------------------ */
+/* nerfacc.pack_info (NA/nerfacc/pack.py:46-77) over the first *n_dev samples. */
+int sn_pack_info_dev(const int64_t* ray_indices, int64_t n_cap, const int32_t* n_dev,
+                     int32_t n_rays, int32_t* packed_info, int32_t* n_bad, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* Compaction by a keep mask (ray_marching.py:216-220) over *n_dev samples. */
+int sn_compact_samples_dev(const uint8_t* keep, int32_t n_cap, const int32_t* n_dev,
+                           const int64_t* ray_indices, const float* t_starts,
+                           const float* t_ends, int64_t* out_ray_indices, float* out_t_starts,
+                           float* out_t_ends, int32_t* n_out, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
+/* Visibility pre-pass points (renderer.py:80-100): for the marched centre   */
+/* samples, the start points [n] followed by the end points of the samples  */
+/* whose end is not the next sample's start on the same ray [n_diff].       */
+/* diff_idx[s] = slot of sample s's end point in that tail, or -1.          */
+/* *n_diff and *n_eval = n + n_diff are written on the device.              */
+/* rays_o_c/rays_d_c: centre-ray origins/directions with row stride         */
+/* `centre_stride` floats (9*3 for 3x3 patches read in place).              */
+size_t sn_step_workspace_bytes(int64_t n_cap);
+int sn_centre_points(const int64_t* ray_indices, const float* t_starts, const float* t_ends,
+                     int64_t n_cap, const int32_t* n_dev, const float* rays_o_c,
+                     const float* rays_d_c, int32_t centre_stride, int32_t* diff_idx,
+                     int32_t* n_diff, float* points, int32_t* n_eval, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* NeuS alpha of each centre sample from the pre-pass SDF values           */
+/* (renderer.py:102-118) and the transmittance test of                      */
+/* ray_marching.py:193-215: keep[s] = T_before(s) >= early_stop_eps.        */
+int sn_alpha_visibility(const int32_t* packed_info, int32_t n_rays, const float* sdf,
+                        const int32_t* diff_idx, int64_t n_cap, const int32_t* n_dev,
+                        const float* inv_s, float early_stop_eps, uint8_t* keep,
+                        void* stream);
+/* Patch expansion (renderer.py:142-159): each centre sample's start (and    */
+/* non-contiguous end) moved onto the patch's marching plane for all P rays  */
+/* of the patch; points laid out [n*P starts | n_diff*P ends].              */
+int sn_patch_points(const int64_t* ray_indices, const float* t_starts, const float* t_ends,
+                    int64_t n_cap, const int32_t* n_dev, const float* rays_o,
+                    const float* rays_d, const float* plane_n, int32_t patch_h,
+                    int32_t patch_w, int32_t* diff_idx, int32_t* n_diff, float* points,
+                    int32_t* n_eval, void* workspace, size_t workspace_bytes, void* stream);
+/* Cells of an occupancy update after warm-up (NA/nerfacc/grid.py:181-194): */
+/* n_cells/4 uniform cells then up to n_cells/4 occupied cells (all of them */
+/* in order when fewer), chosen from device uniforms.  *n_out on device.    */
+size_t sn_occ_sample_workspace_bytes(int32_t n_cells);
+int sn_occ_sample_cells(const uint8_t* binary, int32_t n_cells, const float* u_uniform,
+                        const float* u_select, int64_t* out_idx, int32_t* n_out,
+                        void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -33,9 +33,9 @@ SIGNATURES = {
     "sn_compact_workspace_bytes": (_sz, [_i64]),
     "sn_compact_samples": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "sn_ray_march_count": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
-                                     _f32, _i32, _p, _p]),
+                                     _p, _f32, _i32, _p, _p]),
     "sn_ray_march_fill": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
-                                    _f32, _i32, _p, _p, _p, _p, _p]),
+                                    _p, _f32, _i32, _p, _p, _p, _p, _p]),
     "sn_scatter_add_rows": (C.c_int, [_p, _i64, _i32, _p, _p, _i32, _p]),
     "sn_occ_cell_points": (C.c_int, [_p, _i32, _p, _i32, _i32, _i32, _p, _p, _p]),
     "sn_occ_workspace_bytes": (_sz, [_i32]),
@@ -43,6 +43,7 @@ SIGNATURES = {
     "sn_occ_binarize": (C.c_int, [_p, _i32, _f32, _p, _p, _p, _sz, _p]),
     "sn_occ_nonzero": (C.c_int, [_p, _i32, _p, _p, _p, _sz, _p]),
     "sn_hashgrid_layout": (_i64, [_i32, _i32, _i32, _f32, _p]),
+    "sn_hashgrid_set_aggregation": (None, [_i32]),
     "sn_hashgrid_fwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p]),
     "sn_hashgrid_bwd": (C.c_int, [_p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p,
                                   _p, _p]),
@@ -58,7 +59,19 @@ SIGNATURES = {
     "sn_render_step_waves": (C.c_int, [_i32, _i32]),
     "sn_render_step": (C.c_int, [_i32, _i32, _i32] + [_p] * 16 + [_f32, _f32, _f32, _i32]
                        + [_p] * 8),
-    "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, _f32, _f32, _f32, _i32, _p]),
+    "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
+                               C.c_double, _i32, _p]),
+    "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
+    "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz,
+                                         _p]),
+    "sn_step_workspace_bytes": (_sz, [_i64]),
+    "sn_centre_points": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i32, _p, _p, _p, _p, _p,
+                                   _sz, _p]),
+    "sn_alpha_visibility": (C.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _f32, _p, _p]),
+    "sn_patch_points": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _p, _i32, _i32, _p, _p, _p,
+                                  _p, _p, _sz, _p]),
+    "sn_occ_sample_workspace_bytes": (_sz, [_i32]),
+    "sn_occ_sample_cells": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _sz, _p]),
 }
 
 _lib = None

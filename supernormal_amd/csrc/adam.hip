@@ -9,13 +9,15 @@
 
 namespace sn {
 
+// 1 - beta is formed in double on the host: in fp32, 1 - 0.999f is off by
+// 1.3e-5 relative, which torch (Python floats) does not do.
 struct AdamK {
-  float b1, b2, eps, step_size, inv_sqrt_bc2;
+  float b1, b2, omb1, omb2, eps, step_size, inv_sqrt_bc2;
 };
 
 __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, const AdamK& k) {
-  m = k.b1 * m + (1.f - k.b1) * g;
-  v = k.b2 * v + (1.f - k.b2) * g * g;
+  m = k.b1 * m + k.omb1 * g;
+  v = k.b2 * v + k.omb2 * g * g;
   const float denom = sqrtf(v) * k.inv_sqrt_bc2 + k.eps;
   p = p - k.step_size * (m / denom);
 }
@@ -53,8 +55,8 @@ using namespace sn;
 extern "C" {
 
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
-                 int64_t n, float lr, const float* lr_dev, float beta1, float beta2, float eps,
-                 int32_t step, void* stream) {
+                 int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
+                 double eps, int32_t step, void* stream) {
   clear_error();
   SN_REQUIRE(n >= 0 && step >= 1, "sn_adam_step: bad n=%lld / step=%d", (long long)n, step);
   if (n == 0) return SN_OK;
@@ -63,11 +65,13 @@ int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_a
               (uintptr_t)exp_avg_sq) % 16 == 0,
              "sn_adam_step: buffers must be 16-byte aligned");
   AdamK k;
-  k.b1 = beta1;
-  k.b2 = beta2;
-  k.eps = eps;
-  const double bc1 = 1.0 - std::pow((double)beta1, (double)step);
-  const double bc2 = 1.0 - std::pow((double)beta2, (double)step);
+  k.b1 = (float)beta1;
+  k.b2 = (float)beta2;
+  k.omb1 = (float)(1.0 - beta1);
+  k.omb2 = (float)(1.0 - beta2);
+  k.eps = (float)eps;
+  const double bc1 = 1.0 - std::pow(beta1, (double)step);
+  const double bc2 = 1.0 - std::pow(beta2, (double)step);
   k.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
   k.step_size = 0.f;
   adam_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, as_stream(stream)>>>(

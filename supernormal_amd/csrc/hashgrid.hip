@@ -74,29 +74,70 @@ hash_fwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict_
   }
 }
 
+// Wavefront-aggregated atomic add of F values at table row idx: lanes that
+// target the same row sum their values first and one lane issues the atomics.
+// Every lane of the wavefront must call it (inactive lanes with active=false).
+template <int F>
+__device__ __forceinline__ void wave_row_atomic_add(float* __restrict__ tab, uint32_t idx,
+                                                    const float (&v)[F], bool active) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(active);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lidx = __shfl(idx, leader, 64);
+    const bool mine = active && idx == lidx;
+    const uint64_t same = __ballot(mine);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      float s = mine ? v[f] : 0.f;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == leader) atomicAdd(&tab[(int64_t)lidx * F + f], s);
+    }
+    pending &= ~same;
+  }
+}
+
+// Levels whose cells are larger than ~1/kAggScale of the unit interval put many
+// lanes of a wavefront (consecutive samples of a patch) on the same corners:
+// those levels aggregate across the wavefront before the atomics.
+constexpr float kAggScale = 192.f;
+
 template <int F>
 __global__ void __launch_bounds__(256)
 hash_bwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict__ params,
                 HashGridDesc d, const float* __restrict__ genc, float* __restrict__ gparams,
-                float* __restrict__ gx) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                float* __restrict__ gx, int aggregate) {
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool act = i0 < n;
+  const int64_t i = act ? i0 : 0;  // inactive lanes stay for the wavefront collectives
+  if (!aggregate && !act) return;
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   const float* g = genc + i * (int64_t)(d.n_levels * F);
   float dx[3] = {0.f, 0.f, 0.f};
   for (int l = 0; l < d.n_active; ++l) {
     float gl[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) gl[f] = g[l * F + f];
+    for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
     uint32_t idx[8];
     float w[8], fr[3];
     hg_level(p, d, l, idx, w, fr);
     if (gparams) {
       float* tg = gparams + (int64_t)d.offset[l] * F;
+      if (aggregate && d.scale[l] < kAggScale) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+        for (int c = 0; c < 8; ++c) {
+          float v[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)idx[c] * F + f], w[c] * gl[f]);
+          for (int f = 0; f < F; ++f) v[f] = w[c] * gl[f];
+          wave_row_atomic_add<F>(tg, idx[c], v, act);
+        }
+      } else if (act) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)idx[c] * F + f], w[c] * gl[f]);
+      }
     }
     if (gx) {
       float dw[8][3];
@@ -115,7 +156,7 @@ hash_bwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict_
       for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
     }
   }
-  if (gx) {
+  if (gx && act) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) gx[3 * i + k] = dx[k];
   }
@@ -203,6 +244,8 @@ static int hg_setup(const char* who, int32_t n_levels, int32_t F, int32_t log2_T
   return SN_OK;
 }
 
+static int g_aggregate = 1;
+
 #define SN_DISPATCH_F(F, ...)          \
   switch (F) {                         \
     case 1: { constexpr int kF = 1; __VA_ARGS__; } break; \
@@ -216,6 +259,8 @@ static int hg_setup(const char* who, int32_t n_levels, int32_t F, int32_t log2_T
 using namespace sn;
 
 extern "C" {
+
+void sn_hashgrid_set_aggregation(int32_t on) { g_aggregate = on ? 1 : 0; }
 
 int64_t sn_hashgrid_layout(int32_t n_levels, int32_t log2_hashmap_size, int32_t base_resolution,
                            float per_level_scale, uint32_t* offsets) {
@@ -261,7 +306,7 @@ int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_le
   SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
   SN_DISPATCH_F(n_features,
                 hash_bwd_kernel<kF><<<(unsigned)ceil_div(n, 256), 256, 0, as_stream(stream)>>>(
-                    x, n, params, d, grad_enc, grad_params, grad_x));
+                    x, n, params, d, grad_enc, grad_params, grad_x, g_aggregate));
   return check_launch("sn_hashgrid_bwd");
 }
 

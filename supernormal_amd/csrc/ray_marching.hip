@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(64)
 ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restrict__ rays_d,
                      const float* __restrict__ t_min, const float* __restrict__ t_max,
                      int32_t n_rays, const float* __restrict__ roi, int32_t rx, int32_t ry,
-                     int32_t rz, const uint8_t* __restrict__ grid, float step, float cone,
+                     int32_t rz, const uint8_t* __restrict__ grid, float step,
+                     const float* __restrict__ step_dev, float cone,
                      int32_t max_per_ray, int32_t* __restrict__ num_steps,
                      const int32_t* __restrict__ packed_info, int64_t* __restrict__ ray_indices,
                      float* __restrict__ t_starts, float* __restrict__ t_ends);
@@ -80,7 +81,8 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
                          const float* __restrict__ t_min, const float* __restrict__ t_max,
                          int32_t n_rays, const float* __restrict__ roi, int32_t rx,
                          int32_t ry, int32_t rz, const uint8_t* __restrict__ grid, float step,
-                         float cone, int32_t max_per_ray, int32_t* __restrict__ num_steps,
+                         const float* __restrict__ step_dev, float cone, int32_t max_per_ray,
+                         int32_t* __restrict__ num_steps,
                          const int32_t* __restrict__ packed_info,
                          int64_t* __restrict__ ray_indices, float* __restrict__ t_starts,
                          float* __restrict__ t_ends) {
@@ -103,7 +105,12 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
     inv_dir[d] = 1.0f / dir[d];
   }
   const float near = t_min[i], far = t_max[i];
-  const float dt_min = step, dt_max = 1e10f;
+  // the step may live in device memory (set per iteration without a host sync)
+  const float dt_min = step_dev ? *step_dev : step, dt_max = 1e10f;
+  if (!(dt_min > 0.f)) {
+    if (!kFill) num_steps[i] = 0;
+    return;
+  }
   int64_t base = 0;
   int32_t cap = max_per_ray > 0 ? max_per_ray : 0x7fffffff;
   if (kFill) {
@@ -167,36 +174,36 @@ extern "C" {
 int sn_ray_march_count(const float* rays_o, const float* rays_d, const float* t_min,
                        const float* t_max, int32_t n_rays, const float* roi_aabb,
                        const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
-                       float step_size, float cone_angle, int32_t max_per_ray,
-                       int32_t* num_steps, void* stream) {
+                       float step_size, const float* step_dev, float cone_angle,
+                       int32_t max_per_ray, int32_t* num_steps, void* stream) {
   clear_error();
   int rc = check_march_args("sn_ray_march_count", rays_o, rays_d, t_min, t_max, n_rays,
-                            roi_aabb, grid, rx, ry, rz, step_size, cone_angle);
+                            roi_aabb, grid, rx, ry, rz, step_dev ? 1.f : step_size, cone_angle);
   if (rc) return rc;
   if (n_rays == 0) return SN_OK;
   SN_REQUIRE(num_steps, "sn_ray_march_count: null num_steps");
   ray_march_kernel_roi<false><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
-      rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, cone_angle,
-      max_per_ray, num_steps, nullptr, nullptr, nullptr, nullptr);
+      rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, step_dev,
+      cone_angle, max_per_ray, num_steps, nullptr, nullptr, nullptr, nullptr);
   return check_launch("sn_ray_march_count");
 }
 
 int sn_ray_march_fill(const float* rays_o, const float* rays_d, const float* t_min,
                       const float* t_max, int32_t n_rays, const float* roi_aabb,
                       const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
-                      float step_size, float cone_angle, int32_t max_per_ray,
-                      const int32_t* packed_info, int64_t* ray_indices, float* t_starts,
-                      float* t_ends, void* stream) {
+                      float step_size, const float* step_dev, float cone_angle,
+                      int32_t max_per_ray, const int32_t* packed_info, int64_t* ray_indices,
+                      float* t_starts, float* t_ends, void* stream) {
   clear_error();
   int rc = check_march_args("sn_ray_march_fill", rays_o, rays_d, t_min, t_max, n_rays,
-                            roi_aabb, grid, rx, ry, rz, step_size, cone_angle);
+                            roi_aabb, grid, rx, ry, rz, step_dev ? 1.f : step_size, cone_angle);
   if (rc) return rc;
   if (n_rays == 0) return SN_OK;
   SN_REQUIRE(packed_info && ray_indices && t_starts && t_ends,
              "sn_ray_march_fill: null output");
   ray_march_kernel_roi<true><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
-      rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, cone_angle,
-      max_per_ray, nullptr, packed_info, ray_indices, t_starts, t_ends);
+      rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, step_dev,
+      cone_angle, max_per_ray, nullptr, packed_info, ray_indices, t_starts, t_ends);
   return check_launch("sn_ray_march_fill");
 }
 

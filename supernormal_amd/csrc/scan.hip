@@ -1,12 +1,16 @@
 // Device-wide exclusive scans: pack_info, counts -> packed_info, stream
-// compaction of marching samples, and the small fill kernel used instead of
-// hipMemsetAsync (which misbehaves inside captured graphs).
+// compaction of marching samples, occupied-cell lists, and the small fill
+// kernels used instead of hipMemsetAsync (which misbehaves inside captured
+// graphs).
 //
 // Scan layout: a workgroup of 256 lanes owns a tile of 1024 consecutive
-// items (4 per lane, contiguous per lane so loads are dwordx4-friendly);
-// pass 1 writes one total per tile, pass 2 scans the tile totals in a single
-// workgroup, pass 3 rescans each tile and adds its offset.  All integer, so
-// the result is exact and independent of the launch shape.
+// items (4 per lane); pass 1 writes one total per tile, pass 2 scans the tile
+// totals in one workgroup, pass 3 rescans each tile and adds its offset.  All
+// integer, so the result is exact and independent of the launch shape.
+//
+// Every kernel takes a capacity n_cap (which sizes the launch) and an
+// optional device count n_dev (the number of valid items, <= n_cap), so the
+// scans run without a host round trip inside a sync-free iteration.
 #include "sn_common.h"
 
 namespace sn {
@@ -14,6 +18,12 @@ namespace sn {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int64_t eff_n(int64_t n_cap, const int32_t* n_dev) {
+  if (!n_dev) return n_cap;
+  const int64_t v = *n_dev;
+  return v < 0 ? 0 : (v < n_cap ? v : n_cap);
+}
 
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -82,7 +92,8 @@ struct U8Src {
 
 template <typename Src>
 __global__ void __launch_bounds__(kScanBlock)
-tile_totals_kernel(Src src, int64_t n, int32_t* tile_tot) {
+tile_totals_kernel(Src src, int64_t n_cap, const int32_t* n_dev, int32_t* tile_tot) {
+  const int64_t n = eff_n(n_cap, n_dev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int32_t s = 0;
 #pragma unroll
@@ -93,10 +104,11 @@ tile_totals_kernel(Src src, int64_t n, int32_t* tile_tot) {
   if (threadIdx.x == 0) tile_tot[blockIdx.x] = tot;
 }
 
-// Single workgroup: exclusive scan of n_tiles totals in place, grand total to
+// Single workgroup: exclusive scan of the tile totals in place, grand total to
 // tile_tot[n_tiles] and (optionally) *total_out.
 __global__ void __launch_bounds__(kScanBlock)
-scan_tiles_kernel(int32_t* tile_tot, int64_t n_tiles, int32_t* total_out) {
+scan_tiles_kernel(int32_t* tile_tot, int64_t n_cap, const int32_t* n_dev, int32_t* total_out) {
+  const int64_t n_tiles = ceil_div(n_cap, kScanTile);
   int32_t carry = 0;
   for (int64_t c = 0; c < n_tiles; c += kScanTile) {
     const int64_t base = c + threadIdx.x * kScanItems;
@@ -121,6 +133,7 @@ scan_tiles_kernel(int32_t* tile_tot, int64_t n_tiles, int32_t* total_out) {
     tile_tot[n_tiles] = carry;
     if (total_out) *total_out = carry;
   }
+  (void)n_dev;
 }
 
 // Pass 3 writers.
@@ -135,10 +148,17 @@ struct OffsetDst {
   int32_t* p;
   __device__ void operator()(int64_t i, int32_t excl, int32_t) const { p[i] = excl; }
 };
+// offset of flagged items, -1 elsewhere
+struct FlagOffsetDst {
+  int32_t* p;
+  __device__ void operator()(int64_t i, int32_t excl, int32_t v) const { p[i] = v ? excl : -1; }
+};
 
 template <typename Src, typename Dst>
 __global__ void __launch_bounds__(kScanBlock)
-tile_scan_write_kernel(Src src, int64_t n, const int32_t* tile_off, Dst dst) {
+tile_scan_write_kernel(Src src, int64_t n_cap, const int32_t* n_dev, const int32_t* tile_off,
+                       Dst dst) {
+  const int64_t n = eff_n(n_cap, n_dev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int32_t v[kScanItems];
   int32_t s = 0;
@@ -162,26 +182,35 @@ size_t scan_ws_bytes(int64_t n) {
 }
 
 template <typename Src, typename Dst>
-int exclusive_scan(Src src, int64_t n, Dst dst, int32_t* total, void* ws, size_t ws_bytes,
-                   hipStream_t s, const char* what) {
-  if (ws_bytes < scan_ws_bytes(n)) {
-    set_error("%s: workspace %zu < %zu bytes", what, ws_bytes, scan_ws_bytes(n));
+int exclusive_scan(Src src, int64_t n_cap, const int32_t* n_dev, Dst dst, int32_t* total,
+                   void* ws, size_t ws_bytes, hipStream_t s, const char* what) {
+  if (ws_bytes < scan_ws_bytes(n_cap)) {
+    set_error("%s: workspace %zu < %zu bytes", what, ws_bytes, scan_ws_bytes(n_cap));
     return SN_ESPACE;
   }
   int32_t* tiles = reinterpret_cast<int32_t*>(ws);
-  int64_t n_tiles = ceil_div(n, kScanTile);
-  if (n <= 0) {
+  if (n_cap <= 0) {
     if (total) return fill_i32(total, 1, 0, s);
     return SN_OK;
   }
-  tile_totals_kernel<<<(unsigned)n_tiles, kScanBlock, 0, s>>>(src, n, tiles);
-  scan_tiles_kernel<<<1, kScanBlock, 0, s>>>(tiles, n_tiles, total);
-  tile_scan_write_kernel<<<(unsigned)n_tiles, kScanBlock, 0, s>>>(src, n, tiles, dst);
+  const int64_t n_tiles = ceil_div(n_cap, kScanTile);
+  tile_totals_kernel<<<(unsigned)n_tiles, kScanBlock, 0, s>>>(src, n_cap, n_dev, tiles);
+  scan_tiles_kernel<<<1, kScanBlock, 0, s>>>(tiles, n_cap, n_dev, total);
+  tile_scan_write_kernel<<<(unsigned)n_tiles, kScanBlock, 0, s>>>(src, n_cap, n_dev, tiles, dst);
   return check_launch(what);
 }
 
-__global__ void count_indices_kernel(const int64_t* idx, int64_t n, int32_t n_rays,
-                                     int32_t* counts, int32_t* n_bad) {
+// offsets[i] = number of flagged items before i when flags[i] != 0, else -1;
+// *total = number of flagged items (device).  Used by the sync-free iteration.
+int scan_flags_dev(const uint8_t* flags, int64_t n_cap, const int32_t* n_dev, int32_t* offsets,
+                   int32_t* total, void* ws, size_t ws_bytes, hipStream_t s) {
+  return exclusive_scan(U8Src{flags}, n_cap, n_dev, FlagOffsetDst{offsets}, total, ws, ws_bytes,
+                        s, "scan_flags_dev");
+}
+
+__global__ void count_indices_kernel(const int64_t* idx, int64_t n_cap, const int32_t* n_dev,
+                                     int32_t n_rays, int32_t* counts, int32_t* n_bad) {
+  const int64_t n = eff_n(n_cap, n_dev);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     int64_t r = idx[i];
@@ -192,10 +221,10 @@ __global__ void count_indices_kernel(const int64_t* idx, int64_t n, int32_t n_ra
   }
 }
 
-template <typename T>
-__global__ void compact_kernel(const uint8_t* keep, int64_t n, const int32_t* offs,
-                               const int64_t* ri, const float* t0, const float* t1,
-                               int64_t* o_ri, float* o_t0, float* o_t1) {
+__global__ void compact_kernel(const uint8_t* keep, int64_t n_cap, const int32_t* n_dev,
+                               const int32_t* offs, const int64_t* ri, const float* t0,
+                               const float* t1, int64_t* o_ri, float* o_t0, float* o_t1) {
+  const int64_t n = eff_n(n_cap, n_dev);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (keep[i]) {
@@ -208,7 +237,7 @@ __global__ void compact_kernel(const uint8_t* keep, int64_t n, const int32_t* of
 }
 
 __global__ void nonzero_write_kernel(const uint8_t* keep, int64_t n, const int32_t* offs,
-                                    int64_t* out) {
+                                     int64_t* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     if (keep[i]) out[offs[i]] = i;
@@ -227,8 +256,8 @@ int sn_counts_to_packed_info(const int32_t* counts, int32_t n, int32_t* packed_i
   clear_error();
   SN_REQUIRE(n >= 0, "sn_counts_to_packed_info: n=%d < 0", n);
   SN_REQUIRE(n == 0 || (counts && packed_info), "sn_counts_to_packed_info: null pointer");
-  return exclusive_scan(I32Src{counts}, n, PackedInfoDst{packed_info}, total, ws, ws_bytes,
-                        as_stream(stream), "sn_counts_to_packed_info");
+  return exclusive_scan(I32Src{counts}, n, nullptr, PackedInfoDst{packed_info}, total, ws,
+                        ws_bytes, as_stream(stream), "sn_counts_to_packed_info");
 }
 
 size_t sn_pack_info_workspace_bytes(int32_t n_rays) {
@@ -236,12 +265,12 @@ size_t sn_pack_info_workspace_bytes(int32_t n_rays) {
          scan_ws_bytes(n_rays);
 }
 
-int sn_pack_info(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
-                 int32_t* packed_info, int32_t* n_bad, void* ws, size_t ws_bytes,
-                 void* stream) {
+int sn_pack_info_dev(const int64_t* ray_indices, int64_t n_cap, const int32_t* n_dev,
+                     int32_t n_rays, int32_t* packed_info, int32_t* n_bad, void* ws,
+                     size_t ws_bytes, void* stream) {
   clear_error();
-  SN_REQUIRE(n_rays >= 0 && n_samples >= 0, "sn_pack_info: negative size");
-  SN_REQUIRE(n_samples == 0 || ray_indices, "sn_pack_info: null ray_indices");
+  SN_REQUIRE(n_rays >= 0 && n_cap >= 0, "sn_pack_info: negative size");
+  SN_REQUIRE(n_cap == 0 || ray_indices, "sn_pack_info: null ray_indices");
   SN_REQUIRE(n_rays == 0 || packed_info, "sn_pack_info: null packed_info");
   if (ws_bytes < sn_pack_info_workspace_bytes(n_rays)) {
     set_error("sn_pack_info: workspace %zu < %zu bytes", ws_bytes,
@@ -254,41 +283,57 @@ int sn_pack_info(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
   int rc = fill_i32(counts, n_rays, 0, s);
   if (rc) return rc;
   if (n_bad && (rc = fill_i32(n_bad, 1, 0, s))) return rc;
-  if (n_samples > 0) {
-    count_indices_kernel<<<grid_for(n_samples, 256), 256, 0, s>>>(ray_indices, n_samples,
-                                                                  n_rays, counts, n_bad);
+  if (n_cap > 0) {
+    count_indices_kernel<<<grid_for(n_cap, 256), 256, 0, s>>>(ray_indices, n_cap, n_dev,
+                                                              n_rays, counts, n_bad);
     if ((rc = check_launch("sn_pack_info/count"))) return rc;
   }
-  return exclusive_scan(I32Src{counts}, n_rays, PackedInfoDst{packed_info}, nullptr,
+  return exclusive_scan(I32Src{counts}, n_rays, nullptr, PackedInfoDst{packed_info}, nullptr,
                         (char*)ws + cbytes, ws_bytes - cbytes, s, "sn_pack_info");
+}
+
+int sn_pack_info(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
+                 int32_t* packed_info, int32_t* n_bad, void* ws, size_t ws_bytes,
+                 void* stream) {
+  return sn_pack_info_dev(ray_indices, n_samples, nullptr, n_rays, packed_info, n_bad, ws,
+                          ws_bytes, stream);
 }
 
 size_t sn_compact_workspace_bytes(int64_t n) {
   return align_up((size_t)(n > 0 ? n : 1) * sizeof(int32_t), 256) + scan_ws_bytes(n);
 }
 
-int sn_compact_samples(const uint8_t* keep, int32_t n, const int64_t* ray_indices,
-                       const float* t_starts, const float* t_ends, int64_t* out_ray_indices,
-                       float* out_t_starts, float* out_t_ends, int32_t* n_out, void* ws,
-                       size_t ws_bytes, void* stream) {
+int sn_compact_samples_dev(const uint8_t* keep, int32_t n_cap, const int32_t* n_dev,
+                           const int64_t* ray_indices, const float* t_starts,
+                           const float* t_ends, int64_t* out_ray_indices, float* out_t_starts,
+                           float* out_t_ends, int32_t* n_out, void* ws, size_t ws_bytes,
+                           void* stream) {
   clear_error();
-  SN_REQUIRE(n >= 0, "sn_compact_samples: n < 0");
+  SN_REQUIRE(n_cap >= 0, "sn_compact_samples: n < 0");
   SN_REQUIRE(n_out, "sn_compact_samples: null n_out");
-  if (ws_bytes < sn_compact_workspace_bytes(n)) {
+  if (ws_bytes < sn_compact_workspace_bytes(n_cap)) {
     set_error("sn_compact_samples: workspace %zu < %zu bytes", ws_bytes,
-              sn_compact_workspace_bytes(n));
+              sn_compact_workspace_bytes(n_cap));
     return SN_ESPACE;
   }
   hipStream_t s = as_stream(stream);
   int32_t* offs = reinterpret_cast<int32_t*>(ws);
-  size_t obytes = align_up((size_t)(n > 0 ? n : 1) * sizeof(int32_t), 256);
-  int rc = exclusive_scan(U8Src{keep}, n, OffsetDst{offs}, n_out, (char*)ws + obytes,
-                          ws_bytes - obytes, s, "sn_compact_samples/scan");
-  if (rc || n == 0) return rc;
-  compact_kernel<int><<<grid_for(n, 256), 256, 0, s>>>(keep, n, offs, ray_indices, t_starts,
-                                                       t_ends, out_ray_indices, out_t_starts,
-                                                       out_t_ends);
+  size_t obytes = align_up((size_t)(n_cap > 0 ? n_cap : 1) * sizeof(int32_t), 256);
+  int rc = exclusive_scan(U8Src{keep}, n_cap, n_dev, OffsetDst{offs}, n_out,
+                          (char*)ws + obytes, ws_bytes - obytes, s, "sn_compact_samples/scan");
+  if (rc || n_cap == 0) return rc;
+  compact_kernel<<<grid_for(n_cap, 256), 256, 0, s>>>(keep, n_cap, n_dev, offs, ray_indices,
+                                                      t_starts, t_ends, out_ray_indices,
+                                                      out_t_starts, out_t_ends);
   return check_launch("sn_compact_samples");
+}
+
+int sn_compact_samples(const uint8_t* keep, int32_t n, const int64_t* ray_indices,
+                       const float* t_starts, const float* t_ends, int64_t* out_ray_indices,
+                       float* out_t_starts, float* out_t_ends, int32_t* n_out, void* ws,
+                       size_t ws_bytes, void* stream) {
+  return sn_compact_samples_dev(keep, n, nullptr, ray_indices, t_starts, t_ends, out_ray_indices,
+                                out_t_starts, out_t_ends, n_out, ws, ws_bytes, stream);
 }
 
 int sn_occ_nonzero(const uint8_t* binary, int32_t n_cells, int64_t* out_idx, int32_t* count,
@@ -304,8 +349,8 @@ int sn_occ_nonzero(const uint8_t* binary, int32_t n_cells, int64_t* out_idx, int
   hipStream_t s = as_stream(stream);
   int32_t* offs = reinterpret_cast<int32_t*>(ws);
   size_t obytes = align_up((size_t)(n_cells > 0 ? n_cells : 1) * sizeof(int32_t), 256);
-  int rc = exclusive_scan(U8Src{binary}, n_cells, OffsetDst{offs}, count, (char*)ws + obytes,
-                          ws_bytes - obytes, s, "sn_occ_nonzero/scan");
+  int rc = exclusive_scan(U8Src{binary}, n_cells, nullptr, OffsetDst{offs}, count,
+                          (char*)ws + obytes, ws_bytes - obytes, s, "sn_occ_nonzero/scan");
   if (rc || n_cells == 0) return rc;
   nonzero_write_kernel<<<grid_for(n_cells, 256), 256, 0, s>>>(binary, n_cells, offs, out_idx);
   return check_launch("sn_occ_nonzero");

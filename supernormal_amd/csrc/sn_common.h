@@ -41,6 +41,19 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 256 * 16) {
   return (unsigned)b;
 }
 
+// scan.hip
+size_t scan_ws_bytes(int64_t n);
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+int fill_f32(float* p, int64_t n, float v, hipStream_t s);
+int scan_flags_dev(const uint8_t* flags, int64_t n_cap, const int32_t* n_dev, int32_t* offsets,
+                   int32_t* total, void* ws, size_t ws_bytes, hipStream_t s);
+
+__device__ __forceinline__ int64_t dev_count(int64_t n_cap, const int32_t* n_dev) {
+  if (!n_dev) return n_cap;
+  const int64_t v = *n_dev;
+  return v < 0 ? 0 : (v < n_cap ? v : n_cap);
+}
+
 }  // namespace sn
 
 #define SN_REQUIRE(cond, ...)      \

@@ -423,7 +423,8 @@ def march_raw(rays_o: Tensor, rays_d: Tensor, t_min: Tensor, t_max: Tensor, roi:
     packed = torch.empty((n_rays, 2), dtype=torch.int32, device=dev)
     total = torch.zeros(1, dtype=torch.int32, device=dev)
     args = (ptr(rays_o), ptr(rays_d), ptr(t_min), ptr(t_max), n_rays, ptr(roi), ptr(binary),
-            rx, ry, rz, float(render_step_size), float(cone_angle), int(max_samples_per_ray))
+            rx, ry, rz, float(render_step_size), None, float(cone_angle),
+            int(max_samples_per_ray))
     if n_rays:
         call("sn_ray_march_count", *args, ptr(counts), s)
         ws = workspace(lib.sn_scan_workspace_bytes(n_rays), dev)

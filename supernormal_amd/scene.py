@@ -311,13 +311,17 @@ class Dataset:
                 continue
             pick = m[torch.randint(0, m.numel(), (per,), generator=g)]
             ys, xs = pick // self.W, pick % self.W
-            pts.append(_first_hits(self.scene, self.intrinsics_all[i].cpu().numpy(),
-                                   self.pose_all[i].cpu().numpy(), xs, ys, self.device))
+            x, ok = _first_hits(self.scene, self.intrinsics_all[i].cpu().numpy(),
+                                self.pose_all[i].cpu().numpy(), xs, ys, self.device)
+            pts.append(x[ok])
         return torch.cat(pts).cpu().numpy()
 
 
 @torch.no_grad()
-def _first_hits(scene, K, pose, xs, ys, device, n_iter=200):
+def _first_hits(scene, K, pose, xs, ys, device, n_iter=200, tol=1e-5):
+    """Sphere tracing of the pixel rays inside the unit sphere; returns the
+    points and a mask of the rays that converged onto the surface (a ray that
+    grazes an edge the mask covers can leave the sphere without a hit)."""
     Kinv = torch.inverse(torch.as_tensor(K[:3, :3], dtype=torch.float64, device=device))
     R = torch.as_tensor(pose[:3, :3], dtype=torch.float64, device=device)
     o = torch.as_tensor(pose[:3, 3], dtype=torch.float64, device=device)
@@ -327,10 +331,13 @@ def _first_hits(scene, K, pose, xs, ys, device, n_iter=200):
     d = d @ R.T
     b = 2 * (o * d).sum(-1)
     c = (o * o).sum() - 1.0
-    t = ((-b - (b * b - 4 * c).clamp_min(0).sqrt()) / 2).clamp_min(0)
+    disc = (b * b - 4 * c).clamp_min(0).sqrt()
+    t = ((-b - disc) / 2).clamp_min(0)
+    t_far = (-b + disc) / 2
     for _ in range(n_iter):
-        t = t + scene(o + d * t[:, None]) * 0.9
-    return o + d * t[:, None]
+        t = torch.minimum(t + scene(o + d * t[:, None]) * 0.9, t_far)
+    x = o + d * t[:, None]
+    return x, scene(x).abs() < tol
 
 
 def config_dataset(name: str, device, seed: int = 0) -> Dataset:

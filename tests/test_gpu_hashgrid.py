@@ -65,6 +65,27 @@ def test_forward_backward_vs_fp64(name, n_active, dev):
     torch.testing.assert_close(xt.grad.cpu().double(), x64.grad, rtol=1e-4, atol=2e-4)
 
 
+def test_backward_aggregation_matches_plain_atomics(dev):
+    """The wavefront-aggregated backward (coarse levels) equals plain atomics
+    within fp32 reassociation, on clustered points like a patch's samples."""
+    from supernormal_amd import _lib
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=5)
+    g = torch.Generator().manual_seed(6)
+    centers = torch.rand(300, 3, generator=g) * 1.6 - 0.8
+    x = (centers[:, None, :] + torch.randn(300, 64, 3, generator=g) * 0.004).reshape(-1, 3)
+    x = x.to(dev)
+    genc = torch.randn(x.shape[0], e.n_output_dims, generator=g).to(dev)
+    grads = []
+    for on in (1, 0):
+        _lib.lib().sn_hashgrid_set_aggregation(on)
+        e.params.grad = None
+        (e(x) * genc).sum().backward()
+        grads.append(e.params.grad.clone())
+    _lib.lib().sn_hashgrid_set_aggregation(1)
+    torch.testing.assert_close(grads[0], grads[1], rtol=1e-4, atol=1e-4)
+
+
 def test_double_backward_vs_fp64(dev):
     """L = sum(v * dL1/dx) with dL1/dx = autograd.grad(sum(g * enc), x, create_graph=True):
     d L / d params, d L / d g (through enc's upstream gradient) and d L / d x."""
