@@ -7,7 +7,10 @@ x 2 features, T = 2^19, base 16, x1.38; SDF MLP 64 wide, 2 hidden layers;
 2048 patches of 3x3 (18 432 rays) per iteration; occupancy grid 128^3; <= 128
 samples per ray; 5000 iterations with the reference's schedules.  A "step" is
 one training iteration (occupancy update when due, patch sampling, marching,
-rendering, losses, backward, Adam) on the fused HIP path.
+rendering, losses, backward, Adam) on the fused HIP path: by default with no
+host round trip and replayed as one HIP graph (--mode graph), the MLP on the
+fp16/bf16 matrix cores with fp32 accumulation and fp32 master weights
+(--precision fp16).
 
 Timed region: W untimed warm-up iterations, then K iterations bracketed by
 device synchronisation; the default W + K = 5000 covers the whole schedule,
@@ -15,7 +18,9 @@ so `value` is the run-average iterations/s.  Beside it:
   * plain_gpu: the reference-structured path (torch autograd over the drop-in
     kernels, torch Linear MLP) timed from the same trained state;
   * roofline: the dominant kernel's algorithmic bytes per launch / its mean
-    duration (HIP events on the launch stream over the timed region);
+    duration (HIP events on the launch stream; in graph mode the kernels run
+    inside the replayed graph, so they are timed over --roofline-iters eager
+    iterations of the same path right after the timed region);
   * cpu_baseline: the CPU oracle (torch restatement of the reference
     iteration) on the same workload, a bounded number of iterations;
   * accuracy: Chamfer distance / F-score of the visible surface points of the
@@ -41,20 +46,16 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def hash_bwd_bytes(args, grid):
-    """Algorithmic bytes of one sn_hashgrid_bwd launch: positions (12 B) and
-    dL/denc (L*F*4 B) read per point, plus an fp32 read-modify-write of the 8
-    corners x F features of every active level."""
-    n = int(args[1])
-    n_active = int(args[8])
+def hash_bwd_bytes(n, n_active, grid):
+    """Algorithmic bytes of one hash-grid backward launch over n points:
+    positions (12 B) and dL/denc (L*F*4 B) read per point, plus an fp32
+    read-modify-write of the 8 corners x F features of every active level."""
     L, F = grid["n_levels"], grid["n_features"]
     return n * (12 + L * F * 4 + n_active * 8 * F * 4 * 2)
 
 
-def hash_fwd_bytes(args, grid):
+def hash_fwd_bytes(n, n_active, grid):
     """Positions read, 8 corners x F fp32 gathered per active level, L*F written."""
-    n = int(args[1])
-    n_active = int(args[8])
     L, F = grid["n_levels"], grid["n_features"]
     return n * (12 + n_active * 8 * F * 4 + L * F * 4)
 
@@ -104,6 +105,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
+    ap.add_argument("--roofline-iters", type=int, default=20)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU design is "
@@ -113,34 +117,62 @@ def main(argv=None):
     from supernormal_amd.runner import Runner, synthetic_conf
 
     dev = torch.device("cuda:0")
+    mode = a.mode if a.method == "dfd" else "fused"
     conf = synthetic_conf(a.workload, gradient_method=a.method,
-                          end_iter=max(a.steps + a.warmup, 1))
+                          end_iter=max(a.steps + a.warmup, 1), mlp_precision=a.precision,
+                          step_mode=mode)
     runner = Runner(conf, fused=True, device=dev)
+    sf = runner.sf_step
     grid = runner.sdf_network.encoding.grid
     runner.update_learning_rate()
     torch.cuda.synchronize()
     for i in range(a.warmup):
         runner.train_iteration(i)
     torch.cuda.synchronize()
-    timer = _lib.time_calls(a.roofline_kernel)
+    timer = _lib.time_calls(a.roofline_kernel) if sf is None else None
     samples = 0
+    samples_dev = torch.zeros(1, dtype=torch.float64, device=dev)
     t0 = time.perf_counter()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for i in range(a.warmup, a.warmup + a.steps):
         runner.train_iteration(i)
-        samples += getattr(runner, "last_samples", 0)
+        if sf is not None:
+            samples_dev += sf.ns
+        else:
+            samples += getattr(runner, "last_samples", 0)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
-    _lib.stop_timing(a.roofline_kernel)
+    if sf is not None:
+        samples = float(samples_dev)
+    byte_fn = ROOFLINE_KERNELS[a.roofline_kernel]
+    if sf is None:
+        _lib.stop_timing(a.roofline_kernel)
+        durs = timer.durations_ms()
+        # (x, n, params, levels, F, log2 T, base, scale, n_active, ...)
+        nbytes = [byte_fn(int(args[1]), int(args[8]), grid) for _, _, args in timer.records]
+    else:
+        # the device-count variant: n is read back after each iteration; in
+        # graph mode the kernels run inside the graph, so the launches are timed
+        # over eager iterations of the same path right after the timed region
+        kname = a.roofline_kernel + "_dev"
+        timer = _lib.time_calls(kname)
+        counts = []
+        runner.step_mode = "sync_free"
+        for i in range(a.roofline_iters):
+            runner.train_iteration(runner.iter_step)
+            torch.cuda.synchronize()
+            counts.append(int(sf.ne))
+        runner.step_mode = mode
+        _lib.stop_timing(kname)
+        durs = timer.durations_ms()
+        # (x, n_cap, n_dev, params, levels, F, log2 T, base, scale, n_active, ...)
+        nbytes = [byte_fn(n, int(args[9]), grid) for n, (_, _, args) in zip(counts, timer.records)]
     ms_per_step = ms / max(a.steps, 1)
     value = 1000.0 / ms_per_step
-    durs = timer.durations_ms()
-    byte_fn = ROOFLINE_KERNELS[a.roofline_kernel]
-    nbytes = [byte_fn(args, grid) for _, _, args in timer.records]
     roof = None
     if durs:
         avg_ms = float(np.mean(durs))
@@ -154,10 +186,13 @@ def main(argv=None):
         "metric": "train iters/s at 18k rays vs plain GPU @ equal Chamfer; p50/p99 job time-to-mesh",
         "value": round(value, 2), "unit": "train iters/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if a.precision == "fp32" else "fp16/bf16 MFMA + f32",
+        "data": "synthetic",
         "config": {"workload": f"{a.workload}: torus+box+3 spheres, 20 views 612x512, hash 16x2 "
                                f"T=2^19 base 16 x1.38, MLP 64x2, 2048 3x3 patches (18432 rays), "
-                               f"occupancy 128^3, <=128 samples/ray, gradient {a.method}",
+                               f"occupancy 128^3, <=128 samples/ray, gradient {a.method}, "
+                               f"mode {mode}, mlp {a.precision}",
                    "global_batch": runner.batch_size * 9, "iterations": a.warmup + a.steps,
                    "parallelism": "single GPU"},
         "samples_per_iter": round(samples / max(a.steps, 1), 1),

@@ -252,6 +252,8 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
 /*   in sdf_ends or -1 when the end is the next sample's start (same patch), */
 /*   sdf_starts [S,P], sdf_ends [Nd,P], grad_starts [S,P,3] ('ad', else null),*/
 /*   inv_s, mask_sum (= sum(mask) + 1e-5) and n_samples (= S) on the device.  */
+/*   sdf_ends = NULL: the ends follow the starts in one buffer (row S*P on,   */
+/*   S read on the device), and so do their gradients in g_sdf_starts.       */
 /* Outputs: dL/dsdf_starts [S,P], dL/dsdf_ends [Nd,P], dL/dgrad [S,P,3]      */
 /* ('ad'), losses[4] = (normal, mask, eikonal, dL/dinv_s) (weighted terms),  */
 /* weight_sum [B,P] and comp_normal [B,P,3] (nullable).  partials needs      */
@@ -269,16 +271,32 @@ int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const fl
                    float* losses, float* partials, float* weight_sum, float* comp_normal,
                    void* stream);
 
+/* The same MLP on the fp16 (forward) / bf16 (backward) matrix cores with  */
+/* fp32 accumulation: weights rounded once to the MFMA type, activations    */
+/* and gradients carried as hi+lo pairs (fields.py:85-99 as sn_mlp_fwd /    */
+/* sn_mlp_bwd; sdf only, no dsdf/dh0).  n_dev as in the device-count       */
+/* variants below (nullable).  g_enc [n,E] / g_x [n,3] nullable.            */
+int sn_mlp16_fwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                 const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                 float beta, float* sdf, void* stream);
+size_t sn_mlp16_bwd_workspace_bytes(int64_t n_cap, int32_t enc_dims, int32_t d_hidden,
+                                    int32_t n_layers);
+int sn_mlp16_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                 const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                 float beta, const float* g_sdf, float* g_enc, float* g_x, float* g_mlp,
+                 int32_t accumulate, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
 /* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
 /* p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps).  `lr` is read from   */
 /* device memory (lr_dev) when non-null so that a captured graph can replay */
-/* with a schedule; else the scalar lr is used.                             */
+/* with a schedule; else the scalar lr is used.  Likewise the 1-based step  */
+/* t is read from step_dev (device int32) when non-null, else `step`.       */
 /* ------------------------------------------------------------------------ */
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                  int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
-                 double eps, int32_t step, void* stream);
+                 double eps, int32_t step, const int32_t* step_dev, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Device-count variants for an iteration with no host round trip.  Each    */
@@ -326,6 +344,22 @@ int sn_patch_points(const int64_t* ray_indices, const float* t_starts, const flo
                     const float* rays_d, const float* plane_n, int32_t patch_h,
                     int32_t patch_w, int32_t* diff_idx, int32_t* n_diff, float* points,
                     int32_t* n_eval, void* workspace, size_t workspace_bytes, void* stream);
+/* Hash-grid encode / backward / double backward over the first *n_dev      */
+/* points of an n_cap-point buffer (tcnn Encoding, as sn_hashgrid_*).       */
+int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
+                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        float* enc, void* stream);
+int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
+                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const float* grad_enc, float* grad_params, float* grad_x, void* stream);
+int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                            const float* params, int32_t n_levels, int32_t n_features,
+                            int32_t log2_hashmap_size, int32_t base_resolution,
+                            float per_level_scale, int32_t n_active_levels,
+                            const float* grad_enc, const float* g_gx, float* grad_grad_enc,
+                            float* grad_params, float* grad_x, void* stream);
 /* Cells of an occupancy update after warm-up (NA/nerfacc/grid.py:181-194): */
 /* n_cells/4 uniform cells then up to n_cells/4 occupied cells (all of them */
 /* in order when fewer), chosen from device uniforms.  *n_out on device.    */

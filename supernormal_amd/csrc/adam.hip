@@ -13,6 +13,7 @@ namespace sn {
 // 1.3e-5 relative, which torch (Python floats) does not do.
 struct AdamK {
   float b1, b2, omb1, omb2, eps, step_size, inv_sqrt_bc2;
+  double b1d, b2d;
 };
 
 __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, const AdamK& k) {
@@ -25,7 +26,12 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, con
 __global__ void __launch_bounds__(256)
 adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
             float* __restrict__ v, int64_t n, AdamK k, float lr_scalar,
-            const float* __restrict__ lr_dev, float bc1) {
+            const float* __restrict__ lr_dev, float bc1, const int32_t* __restrict__ step_dev) {
+  if (step_dev) {  // bias corrections of the step held on the device
+    const double st = (double)*step_dev;
+    bc1 = (float)(1.0 - pow(k.b1d, st));
+    k.inv_sqrt_bc2 = (float)(1.0 / sqrt(1.0 - pow(k.b2d, st)));
+  }
   if (lr_dev) k.step_size = lr_dev[0] / bc1;
   else k.step_size = lr_scalar / bc1;
   const int64_t n4 = n / 4;
@@ -56,9 +62,10 @@ extern "C" {
 
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                  int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
-                 double eps, int32_t step, void* stream) {
+                 double eps, int32_t step, const int32_t* step_dev, void* stream) {
   clear_error();
-  SN_REQUIRE(n >= 0 && step >= 1, "sn_adam_step: bad n=%lld / step=%d", (long long)n, step);
+  SN_REQUIRE(n >= 0 && (step >= 1 || step_dev), "sn_adam_step: bad n=%lld / step=%d",
+             (long long)n, step);
   if (n == 0) return SN_OK;
   SN_REQUIRE(params && grads && exp_avg && exp_avg_sq, "sn_adam_step: null pointer");
   SN_REQUIRE(((uintptr_t)params | (uintptr_t)grads | (uintptr_t)exp_avg |
@@ -70,12 +77,15 @@ int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_a
   k.omb1 = (float)(1.0 - beta1);
   k.omb2 = (float)(1.0 - beta2);
   k.eps = (float)eps;
-  const double bc1 = 1.0 - std::pow(beta1, (double)step);
-  const double bc2 = 1.0 - std::pow(beta2, (double)step);
+  k.b1d = beta1;
+  k.b2d = beta2;
+  const double st = step >= 1 ? (double)step : 1.0;
+  const double bc1 = 1.0 - std::pow(beta1, st);
+  const double bc2 = 1.0 - std::pow(beta2, st);
   k.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
   k.step_size = 0.f;
   adam_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, as_stream(stream)>>>(
-      params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1);
+      params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1, step_dev);
   return check_launch("sn_adam_step");
 }
 

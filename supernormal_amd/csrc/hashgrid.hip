@@ -48,10 +48,10 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
 
 template <int F>
 __global__ void __launch_bounds__(256)
-hash_fwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict__ params,
-                HashGridDesc d, float* __restrict__ enc) {
+hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                const float* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= dev_count(n_cap, n_dev)) return;
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   float* out = enc + i * (int64_t)(d.n_levels * F);
   for (int l = 0; l < d.n_levels; ++l) {
@@ -105,10 +105,14 @@ constexpr float kAggScale = 192.f;
 
 template <int F>
 __global__ void __launch_bounds__(256)
-hash_bwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict__ params,
-                HashGridDesc d, const float* __restrict__ genc, float* __restrict__ gparams,
+hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                const float* __restrict__ params, HashGridDesc d,
+                const float* __restrict__ genc, float* __restrict__ gparams,
                 float* __restrict__ gx, int aggregate) {
+  const int64_t n = dev_count(n_cap, n_dev);
   const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // whole wavefronts past the end leave together (uniform branch)
+  if ((i0 & ~(int64_t)63) >= n) return;
   const bool act = i0 < n;
   const int64_t i = act ? i0 : 0;  // inactive lanes stay for the wavefront collectives
   if (!aggregate && !act) return;
@@ -169,12 +173,13 @@ hash_bwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict_
 //                        S_c = sum_f genc[l,f] theta[c,f]  (d2w/dfrac_d^2 = 0)
 template <int F>
 __global__ void __launch_bounds__(256)
-hash_bwd_bwd_kernel(const float* __restrict__ x, int64_t n, const float* __restrict__ params,
+hash_bwd_bwd_kernel(const float* __restrict__ x, int64_t n_cap,
+                    const int32_t* __restrict__ n_dev, const float* __restrict__ params,
                     HashGridDesc d, const float* __restrict__ genc,
                     const float* __restrict__ ggx, float* __restrict__ gge,
                     float* __restrict__ gparams, float* __restrict__ gx2) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= dev_count(n_cap, n_dev)) return;
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   const float v[3] = {ggx[3 * i], ggx[3 * i + 1], ggx[3 * i + 2]};
   float acc2[3] = {0.f, 0.f, 0.f};
@@ -273,41 +278,81 @@ int64_t sn_hashgrid_layout(int32_t n_levels, int32_t log2_hashmap_size, int32_t 
   return tot;
 }
 
-int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_levels,
-                    int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
-                    float per_level_scale, int32_t n_active_levels, float* enc, void* stream) {
+int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
+                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        float* enc, void* stream) {
   clear_error();
   HashGridDesc d;
   int rc = hg_setup("sn_hashgrid_fwd", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
-  SN_REQUIRE(n >= 0, "sn_hashgrid_fwd: n < 0");
-  if (n == 0) return SN_OK;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_fwd: n < 0");
+  if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && enc, "sn_hashgrid_fwd: null pointer");
-  SN_DISPATCH_F(n_features, hash_fwd_kernel<kF><<<(unsigned)ceil_div(n, 256), 256, 0,
-                                                  as_stream(stream)>>>(x, n, params, d, enc));
+  SN_DISPATCH_F(n_features, hash_fwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+                                                  as_stream(stream)>>>(x, n_cap, n_dev, params,
+                                                                       d, enc));
   return check_launch("sn_hashgrid_fwd");
+}
+
+int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_levels,
+                    int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
+                    float per_level_scale, int32_t n_active_levels, float* enc, void* stream) {
+  return sn_hashgrid_fwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
+                             base_resolution, per_level_scale, n_active_levels, enc, stream);
+}
+
+int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
+                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const float* grad_enc, float* grad_params, float* grad_x, void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
+  SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
+  SN_DISPATCH_F(n_features,
+                hash_bwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+                                      as_stream(stream)>>>(x, n_cap, n_dev, params, d, grad_enc,
+                                                           grad_params, grad_x, g_aggregate));
+  return check_launch("sn_hashgrid_bwd");
 }
 
 int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
                     int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
                     float per_level_scale, int32_t n_active_levels, const float* grad_enc,
                     float* grad_params, float* grad_x, void* stream) {
+  return sn_hashgrid_bwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
+                             base_resolution, per_level_scale, n_active_levels, grad_enc,
+                             grad_params, grad_x, stream);
+}
+
+int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                            const float* params, int32_t n_levels, int32_t n_features,
+                            int32_t log2_hashmap_size, int32_t base_resolution,
+                            float per_level_scale, int32_t n_active_levels,
+                            const float* grad_enc, const float* g_gx, float* grad_grad_enc,
+                            float* grad_params, float* grad_x, void* stream) {
   clear_error();
   HashGridDesc d;
-  int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
+  int rc = hg_setup("sn_hashgrid_bwd_bwd", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
-  SN_REQUIRE(n >= 0, "sn_hashgrid_bwd: n < 0");
-  if (n == 0) {
-    return SN_OK;
-  }
-  SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
-  SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
-  SN_DISPATCH_F(n_features,
-                hash_bwd_kernel<kF><<<(unsigned)ceil_div(n, 256), 256, 0, as_stream(stream)>>>(
-                    x, n, params, d, grad_enc, grad_params, grad_x, g_aggregate));
-  return check_launch("sn_hashgrid_bwd");
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd_bwd: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && params && g_gx, "sn_hashgrid_bwd_bwd: null input");
+  SN_REQUIRE(grad_grad_enc || ((grad_params || grad_x) && grad_enc),
+             "sn_hashgrid_bwd_bwd: no output");
+  SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+                                                      as_stream(stream)>>>(
+                                x, n_cap, n_dev, params, d, grad_enc, g_gx, grad_grad_enc,
+                                grad_params, grad_x));
+  return check_launch("sn_hashgrid_bwd_bwd");
 }
 
 int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
@@ -315,20 +360,9 @@ int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t 
                         float per_level_scale, int32_t n_active_levels, const float* grad_enc,
                         const float* g_gx, float* grad_grad_enc, float* grad_params,
                         float* grad_x, void* stream) {
-  clear_error();
-  HashGridDesc d;
-  int rc = hg_setup("sn_hashgrid_bwd_bwd", n_levels, n_features, log2_hashmap_size,
-                    base_resolution, per_level_scale, n_active_levels, &d);
-  if (rc) return rc;
-  SN_REQUIRE(n >= 0, "sn_hashgrid_bwd_bwd: n < 0");
-  if (n == 0) return SN_OK;
-  SN_REQUIRE(x && params && g_gx, "sn_hashgrid_bwd_bwd: null input");
-  SN_REQUIRE(grad_grad_enc || ((grad_params || grad_x) && grad_enc),
-             "sn_hashgrid_bwd_bwd: no output");
-  SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<(unsigned)ceil_div(n, 256), 256, 0,
-                                                      as_stream(stream)>>>(
-                                x, n, params, d, grad_enc, g_gx, grad_grad_enc, grad_params, grad_x));
-  return check_launch("sn_hashgrid_bwd_bwd");
+  return sn_hashgrid_bwd_bwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
+                                 base_resolution, per_level_scale, n_active_levels, grad_enc,
+                                 g_gx, grad_grad_enc, grad_params, grad_x, stream);
 }
 
 }  // extern "C"

@@ -604,14 +604,37 @@ mlp_bwd_ad_kernel(const float* __restrict__ x, const float* __restrict__ enc, co
                            dW0, dW1, db0, db1, dwo, dbo);
 }
 
-__global__ void __launch_bounds__(256)
+// Column sums of the partial rows in a fixed order: a workgroup owns 64
+// columns; its 16 row groups each sum every 16th row, then the 16 group sums
+// are added in order.  Independent of the launch, so results are reproducible.
+constexpr int kRedCols = 64, kRedGroups = 16;
+
+__global__ void __launch_bounds__(kRedCols * kRedGroups)
 reduce_rows_kernel(const float* __restrict__ partials, int64_t rows, int cols, float* __restrict__ out,
                    int accumulate) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+  __shared__ float part[kRedGroups][kRedCols];
+  const int cl = threadIdx.x % kRedCols, g = threadIdx.x / kRedCols;
+  const int c = blockIdx.x * kRedCols + cl;
   float s = 0.f;
-  for (int64_t r = 0; r < rows; ++r) s += partials[r * cols + c];
-  out[c] = accumulate ? out[c] + s : s;
+  if (c < cols) {
+#pragma unroll 8
+    for (int64_t r = g; r < rows; r += kRedGroups) s += partials[r * cols + c];
+  }
+  part[g][cl] = s;
+  __syncthreads();
+  if (g == 0 && c < cols) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < kRedGroups; ++k) t += part[k][cl];
+    out[c] = accumulate ? out[c] + t : t;
+  }
+}
+
+int mlp_reduce_rows(const float* part, int64_t rows, int cols, float* out, int accumulate,
+                    hipStream_t s) {
+  reduce_rows_kernel<<<(unsigned)ceil_div(cols, kRedCols), kRedCols * kRedGroups, 0, s>>>(
+      part, rows, cols, out, accumulate);
+  return check_launch("mlp_reduce_rows");
 }
 
 // ---------------------------------------------------------------------------
@@ -658,12 +681,18 @@ constexpr size_t smem_bytes(int T, int nbuf) {
 // Blocks use more than the default 64 KB of dynamic LDS (gfx950 has 160 KB).
 template <typename Kern>
 static void allow_smem(Kern k, size_t bytes) {
-  static size_t set = 0;
-  if (set < bytes) {
-    // a refusal surfaces as a launch error, reported by check_launch
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    set = bytes;
+  // once per kernel (kernels of one signature share this instantiation, so
+  // the cache is keyed by the kernel); a refusal surfaces as a launch error
+  static const void* done[64];
+  static size_t done_bytes[64];
+  static int n_done = 0;
+  const void* f = reinterpret_cast<const void*>(k);
+  for (int i = 0; i < n_done; ++i)
+    if (done[i] == f && done_bytes[i] >= bytes) return;
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (n_done < 64) {
+    done[n_done] = f;
+    done_bytes[n_done++] = bytes;
   }
 }
 
@@ -739,8 +768,7 @@ int sn_mlp_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap
     mlp_bwd_kernel<kH, kNH, kDP, T><<<grid, 64 * kWaves, sm, s>>>(
         x, enc, enc_dims, n_cap, n_dev, mlp, beta, g_sdf, g_enc, g_x, part);
   });
-  reduce_rows_kernel<<<(unsigned)ceil_div(np, 256), 256, 0, s>>>(part, (int64_t)grid * kWaves,
-                                                                  np, g_mlp, accumulate);
+  if ((rc = mlp_reduce_rows(part, (int64_t)grid * kWaves, np, g_mlp, accumulate, s))) return rc;
   return check_launch("sn_mlp_bwd");
 }
 
@@ -771,8 +799,7 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
     mlp_bwd_ad_kernel<kH, kNH, kDP, T><<<grid, 64 * kWaves, sm, s>>>(
         x, enc, v, enc_dot, enc_dims, n_cap, n_dev, mlp, beta, g_sdf, g_h0, g_h0_dot, part);
   });
-  reduce_rows_kernel<<<(unsigned)ceil_div(np, 256), 256, 0, s>>>(part, (int64_t)grid * kWaves,
-                                                                  np, g_mlp, accumulate);
+  if ((rc = mlp_reduce_rows(part, (int64_t)grid * kWaves, np, g_mlp, accumulate, s))) return rc;
   return check_launch("sn_mlp_bwd_ad");
 }
 

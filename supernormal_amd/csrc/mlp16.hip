@@ -1,0 +1,722 @@
+// The SDF MLP of models/fields.py:85-99 on the fp16/bf16 matrix cores with
+// fp32 accumulation, activations kept in registers between layers.
+//
+// Layout.  Every layer is computed transposed, Z^T = W * H^T: units are the
+// MFMA rows (M) and points the columns (N), 16 points per column block.  With
+// v_mfma_f32_16x16x32 a lane (r = l % 16, q = l / 16) receives column r of
+// the B operand (8 K-slots) and returns rows 4q..4q+3 of column r of D.  So a
+// lane ends a layer holding point r's units {16 mb + 4q + e}, and those are
+// exactly 8 K-slots of the next layer's B operand if the next layer's K axis
+// is ordered
+//     slot (kb, q, j) -> unit 16 (2 kb + j / 4) + 4 q + j % 4 .
+// A dot product does not depend on the order of K as long as A and B agree,
+// so the weights (the A operand) are stored in that order once per launch and
+// no activation goes through LDS in the forward pass.
+//
+// Precision.  Weights are rounded once to the MFMA type (a fixed, smooth
+// perturbation of the network, below 2^-11 relative for fp16); activations
+// and gradients are split into hi + lo parts (x = hi + lo exactly to ~22 bits
+// for fp16, ~16 bits for bf16) and both parts are multiplied, so no per-point
+// rounding noise enters the SDF at fp16 resolution.  The forward pass runs on
+// fp16; the backward on bf16, whose range holds the small loss gradients
+// without scaling.
+//
+// Weight gradients need the points on the K axis, i.e. the transpose of the
+// register layout: each group's gradient and activation blocks are written to
+// a workgroup LDS tile (hi/lo bf16 pairs) and read back as 16x16x16 MFMA
+// operands.  Per-workgroup partial sums are reduced in a fixed order by
+// mlp_reduce_rows, so results do not depend on scheduling.
+//
+// h0 = [x, enc] (input_concat) is held internally as [enc | x | 1 | 0 ...]
+// in KB0 blocks of 32: the constant 1 carries the first bias through the
+// MFMA (and its gradient out of the weight-gradient product).  The packed
+// parameter layout is the fp32 kernels' one:
+//   W0[H][3+E] b0[H] { W1[H][H] b1[H] } wo[H] bo
+#include "sn_common.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+namespace sn {
+namespace m16 {
+
+constexpr int kWaves = 4;
+constexpr float kSpThr = 20.f;
+
+__device__ __forceinline__ floatx4 mma_h(half8 a, half8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mma_b(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mma_b4(bf16x4 a, bf16x4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)v;
+  lo = (_Float16)(v - (float)hi);
+}
+__device__ __forceinline__ void split_b(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+// Softplus(beta) with torch's threshold (value z above it) and its derivative.
+struct Sp {
+  float h, d;
+};
+__device__ __forceinline__ Sp softplus(float z, float beta, float inv_beta) {
+  const float bz = z * beta;
+  Sp s;
+  if (bz > kSpThr) {
+    s.h = z;
+    s.d = 1.f;
+  } else {
+    const float t = __expf(bz);
+    s.h = __logf(1.f + t) * inv_beta;
+    s.d = t / (1.f + t);
+  }
+  return s;
+}
+__device__ __forceinline__ float softplus_h(float z, float beta, float inv_beta) {
+  const float bz = z * beta;
+  return bz > kSpThr ? z : __logf(1.f + __expf(bz)) * inv_beta;
+}
+
+// unit held in slot (kb, q, j) of a hidden-layer K axis
+__host__ __device__ constexpr int hid_unit(int kb, int q, int j) {
+  return 16 * (2 * kb + (j >> 2)) + 4 * q + (j & 3);
+}
+
+template <int H, int NH, int KB0>
+struct Cfg {
+  static constexpr int MB = H / 16;    // 16-unit blocks of a hidden layer
+  static constexpr int KBH = H / 32;   // 32-unit K blocks of a hidden layer
+  static constexpr int MD0 = 2 * KB0;  // 16-dim blocks of h0
+  // LDS fragments (64 lanes x 16 bytes each)
+  static constexpr int nW0 = MB * KB0, nW1 = (NH == 2 ? MB * KBH : 0);
+  static constexpr int nW1T = (NH == 2 ? MB * KBH : 0), nW0T = MD0 * KBH;
+  static constexpr int kVecFloats = 3 * H + 4;  // b0, b1, wo, bo
+  static constexpr size_t fwd_lds = (size_t)(nW0 + nW1) * 64 * 16 + kVecFloats * 4;
+  // two block tiles of 64 rows x (64 points + 4) dwords
+  static constexpr size_t tile_bytes = 2 * 64 * (16 * kWaves + 4) * 4;
+  static constexpr size_t bwd_lds =
+      (size_t)(nW0 + nW1 + nW1T + nW0T) * 64 * 16 + kVecFloats * 4 + tile_bytes;
+  __host__ __device__ static int n_params(int d) {
+    return H * d + H + (NH == 2 ? H * H + H : 0) + H + 1;
+  }
+  __host__ __device__ static int gB0(int d) { return H * d; }
+  __host__ __device__ static int gW1(int d) { return H * d + H; }
+  __host__ __device__ static int gB1(int d) { return H * d + H + H * H; }
+  __host__ __device__ static int gWo(int d) { return H * d + H + (NH == 2 ? H * H + H : 0); }
+  __host__ __device__ static int gBo(int d) { return gWo(d) + H; }
+};
+
+// packed W0 column of internal h0 dim c ([enc | x | 1 | pad]); -2 for the
+// bias dim, -1 for padding
+__device__ __forceinline__ int h0_col(int c, int E) {
+  return c < E ? 3 + c : (c < E + 3 ? c - E : (c == E + 3 ? -2 : -1));
+}
+
+// Builds the A-operand fragments and bias vectors in LDS.
+template <int H, int NH, int KB0, bool kBwd>
+__device__ void build_frags(char* smem, const float* __restrict__ mlp, int E) {
+  using C = Cfg<H, NH, KB0>;
+  const int d = 3 + E;
+  _Float16* fW0 = reinterpret_cast<_Float16*>(smem);
+  _Float16* fW1 = fW0 + C::nW0 * 512;
+  __bf16* fW1T = reinterpret_cast<__bf16*>(fW1 + C::nW1 * 512);
+  __bf16* fW0T = fW1T + (kBwd ? C::nW1T * 512 : 0);
+  float* vec = reinterpret_cast<float*>(fW0T + (kBwd ? C::nW0T * 512 : 0));
+  const float* W0 = mlp;
+  const float* W1 = mlp + C::gW1(d);
+  // element e of a fragment: lane = e / 8 (r = lane % 16, q = lane / 16), slot j = e % 8
+  for (int t = threadIdx.x; t < C::nW0 * 512; t += blockDim.x) {
+    const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
+    const int mb = f / KB0, kb = f % KB0;
+    const int col = h0_col(32 * kb + 8 * q + j, E), o = 16 * mb + r;
+    fW0[t] = (_Float16)(col >= 0 ? W0[o * d + col] : (col == -2 ? mlp[C::gB0(d) + o] : 0.f));
+  }
+  for (int t = threadIdx.x; t < C::nW1 * 512; t += blockDim.x) {
+    const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
+    const int mb = f / C::KBH, kb = f % C::KBH;
+    fW1[t] = (_Float16)W1[(16 * mb + r) * H + hid_unit(kb, q, j)];
+  }
+  if (kBwd) {
+    for (int t = threadIdx.x; t < C::nW1T * 512; t += blockDim.x) {
+      const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
+      const int mi = f / C::KBH, kb = f % C::KBH;
+      fW1T[t] = (__bf16)W1[hid_unit(kb, q, j) * H + 16 * mi + r];
+    }
+    for (int t = threadIdx.x; t < C::nW0T * 512; t += blockDim.x) {
+      const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
+      const int md = f / C::KBH, kb = f % C::KBH;
+      const int col = h0_col(16 * md + r, E);
+      fW0T[t] = (__bf16)(col >= 0 ? W0[hid_unit(kb, q, j) * d + col] : 0.f);
+    }
+  }
+  for (int t = threadIdx.x; t < H; t += blockDim.x) {
+    vec[t] = mlp[C::gB0(d) + t];
+    vec[H + t] = NH == 2 ? mlp[C::gB1(d) + t] : 0.f;
+    vec[2 * H + t] = mlp[C::gWo(d) + t];
+  }
+  if (threadIdx.x == 0) vec[3 * H] = mlp[C::gBo(d)];
+}
+
+// The 8 h0 values of K block kb for point p (lane slots 8q..8q+7).
+__device__ __forceinline__ void load_h0(const float* __restrict__ x, const float* __restrict__ enc,
+                                        int E, bool vec4, int64_t p, bool ok, int kb, int q,
+                                        float (&v)[8]) {
+  const int c0 = 32 * kb + 8 * q;
+  if (ok && vec4 && c0 + 8 <= E) {
+    const float4* s = reinterpret_cast<const float4*>(enc + p * E + c0);
+    const float4 a = s[0], b = s[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = c0 + j;
+    float t = 0.f;
+    if (ok) {
+      if (c < E) t = enc[p * E + c];
+      else if (c < E + 3) t = x[p * 3 + (c - E)];
+      else if (c == E + 3) t = 1.f;
+    }
+    v[j] = t;
+  }
+}
+
+// B operand of a hidden layer from the previous layer's D blocks: slot j of K
+// block kb is unit 16 (2 kb + j/4) + 4q + j%4 = block 2kb + j/4, element j%4.
+template <int MB>
+__device__ __forceinline__ void hid_operand_h(const float (&h)[MB][4], int kb, half8& hi, half8& lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    _Float16 a, b;
+    split_h(h[2 * kb + (j >> 2)][j & 3], a, b);
+    hi[j] = a;
+    lo[j] = b;
+  }
+}
+template <int MB>
+__device__ __forceinline__ void hid_operand_b(const float (&h)[MB][4], int kb, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    __bf16 a, b;
+    split_b(h[2 * kb + (j >> 2)][j & 3], a, b);
+    hi[j] = a;
+    lo[j] = b;
+  }
+}
+
+__device__ __forceinline__ float sum_q(float v) {  // over the 4 lanes of a point (same r)
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+__device__ __forceinline__ float sum_r(float v) {  // over the 16 points of a q group
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+// Forward of one 16-point group: z blocks of every layer (pre-activation) and
+// softplus values/derivatives.  Shared by the forward and backward kernels.
+template <int H, int NH, int KB0>
+struct Fwd {
+  static constexpr int MB = H / 16;
+  float h1[MB][4], d1[MB][4];  // softplus(z0), softplus'(z0)
+  float h2[MB][4], d2[MB][4];  // softplus(z1), softplus'(z1) (NH == 2)
+};
+
+template <int H, int NH, int KB0, bool kDeriv>
+__device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1, const float* vec,
+                                              const float (&h0)[KB0][8], float beta, float inv_beta,
+                                              Fwd<H, NH, KB0>& F) {
+  using C = Cfg<H, NH, KB0>;
+  const int l = threadIdx.x & 63, q = l >> 4;
+  half8 bh[KB0], bl[KB0];
+#pragma unroll
+  for (int kb = 0; kb < KB0; ++kb)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      _Float16 a, b;
+      split_h(h0[kb][j], a, b);
+      bh[kb][j] = a;
+      bl[kb][j] = b;
+    }
+#pragma unroll
+  for (int mb = 0; mb < C::MB; ++mb) {
+    floatx4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB0; ++kb) {
+      const half8 w = fW0[(mb * KB0 + kb) * 64 + l];
+      z = mma_h(w, bl[kb], z);
+      z = mma_h(w, bh[kb], z);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float zz = z[e];  // b0 enters through the constant h0 dim
+      if (kDeriv) {
+        const Sp s = softplus(zz, beta, inv_beta);
+        F.h1[mb][e] = s.h;
+        F.d1[mb][e] = s.d;
+      } else {
+        F.h1[mb][e] = softplus_h(zz, beta, inv_beta);
+      }
+    }
+  }
+  if (NH == 2) {
+    half8 ch[C::KBH], cl[C::KBH];
+#pragma unroll
+    for (int kb = 0; kb < C::KBH; ++kb) hid_operand_h<C::MB>(F.h1, kb, ch[kb], cl[kb]);
+#pragma unroll
+    for (int mb = 0; mb < C::MB; ++mb) {
+      floatx4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < C::KBH; ++kb) {
+        const half8 w = fW1[(mb * C::KBH + kb) * 64 + l];
+        z = mma_h(w, cl[kb], z);
+        z = mma_h(w, ch[kb], z);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float zz = z[e] + vec[H + 16 * mb + 4 * q + e];
+        if (kDeriv) {
+          const Sp s = softplus(zz, beta, inv_beta);
+          F.h2[mb][e] = s.h;
+          F.d2[mb][e] = s.d;
+        } else {
+          F.h2[mb][e] = softplus_h(zz, beta, inv_beta);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward: sdf [n]
+// ---------------------------------------------------------------------------
+template <int H, int NH, int KB0>
+__global__ void __launch_bounds__(64 * kWaves)
+mlp16_fwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int E, int vec4,
+                 int64_t n_cap,
+                 const int32_t* __restrict__ n_dev, const float* __restrict__ mlp, float beta,
+                 float* __restrict__ sdf_out) {
+  using C = Cfg<H, NH, KB0>;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  build_frags<H, NH, KB0, false>(smem, mlp, E);
+  __syncthreads();
+  const half8* fW0 = reinterpret_cast<const half8*>(smem);
+  const half8* fW1 = fW0 + C::nW0 * 64;
+  const float* vec = reinterpret_cast<const float*>(fW1 + C::nW1 * 64);
+  const float inv_beta = 1.f / beta;
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  const int64_t n_groups = ceil_div(n, 16);
+  const int64_t waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t g = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6); g < n_groups; g += waves) {
+    // re-read the weight fragments from LDS every group rather than pinning
+    // them in registers: ~80 VGPRs instead of ~200, three times the waves
+    asm volatile("" ::: "memory");
+    const int64_t p = g * 16 + r;
+    const bool ok = p < n;
+    float h0[KB0][8];
+#pragma unroll
+    for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, E, vec4, p, ok, kb, q, h0[kb]);
+    Fwd<H, NH, KB0> F;
+    forward_group<H, NH, KB0, false>(fW0, fW1, vec, h0, beta, inv_beta, F);
+    float s = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < C::MB; ++mb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        s += vec[2 * H + 16 * mb + 4 * q + e] * (NH == 2 ? F.h2[mb][e] : F.h1[mb][e]);
+    s = sum_q(s) + vec[3 * H];
+    if (q == 0 && ok) sdf_out[p] = s;
+  }
+}
+
+// Block tiles: rows of (hi, lo) bf16 pairs over the 64 points of a block step
+// (wavefront w owns columns 16w..16w+15).  Row pitch 68 dwords keeps the
+// writes of a half-wavefront on distinct banks.
+constexpr int kBlkPts = 16 * kWaves;
+constexpr int kTileP = kBlkPts + 4;
+
+__device__ __forceinline__ uint32_t pack_pair(float v) {
+  __bf16 hi, lo;
+  split_b(v, hi, lo);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+// Writes D-layout blocks (this wavefront's point r, rows 16 mb + 4q + e).
+template <int MB>
+__device__ __forceinline__ void tile_store(uint32_t* t, const float (&v)[MB][4]) {
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  const int col = 16 * (threadIdx.x >> 6) + r;
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) t[(16 * mb + 4 * q + e) * kTileP + col] = pack_pair(v[mb][e]);
+}
+// 16x16x16 operand: tile row `row`, points 16k + 4q .. +3, as hi and lo.
+__device__ __forceinline__ void tile_operand(const uint32_t* t, int row, int k, bf16x4& hi,
+                                             bf16x4& lo) {
+  const int q = (threadIdx.x & 63) >> 4;
+  const uint4 w = *reinterpret_cast<const uint4*>(t + row * kTileP + 16 * k + 4 * q);
+  const uint32_t d[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    hi[j] = __builtin_bit_cast(__bf16, (uint16_t)(d[j] & 0xffffu));
+    lo[j] = __builtin_bit_cast(__bf16, (uint16_t)(d[j] >> 16));
+  }
+}
+// acc += A[16a .. 16a+15][pts] * B[16b .. 16b+15][pts]^T over the step's points.
+__device__ __forceinline__ floatx4 tile_mma(const uint32_t* tA, int a, const uint32_t* tB, int b,
+                                            floatx4 acc) {
+  const int r = threadIdx.x & 15;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) {
+    bf16x4 ah, al, bh, bl;
+    tile_operand(tA, 16 * a + r, k, ah, al);
+    tile_operand(tB, 16 * b + r, k, bh, bl);
+    acc = mma_b4(al, bh, acc);
+    acc = mma_b4(ah, bl, acc);
+    acc = mma_b4(ah, bh, acc);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// backward of sum g_sdf * sdf: one weight-gradient partial row per workgroup
+// (packed layout), g_enc [n, E] and g_x [n, 3] (nullable).
+//
+// A workgroup steps over 64 points at a time (16 per wavefront).  Each
+// wavefront recomputes its group's forward pass and backpropagates it in
+// registers; the step's gradient and activation blocks go to the block tiles
+// and the weight-gradient blocks are split over the wavefronts (block b is
+// accumulated by wavefront b % 4 over all 64 points), so each wavefront holds
+// a quarter of the accumulators.
+// ---------------------------------------------------------------------------
+template <int H, int NH, int KB0>
+__global__ void __launch_bounds__(64 * kWaves, 2)
+mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int E, int vec4,
+                 int64_t n_cap, const int32_t* __restrict__ n_dev, const float* __restrict__ mlp,
+                 float beta, const float* __restrict__ g_sdf, float* __restrict__ g_enc,
+                 float* __restrict__ g_x, float* __restrict__ partials) {
+  using C = Cfg<H, NH, KB0>;
+  constexpr int MB = C::MB, KBH = C::KBH, MD0 = C::MD0;
+  constexpr int NB1 = NH == 2 ? MB * MB : 0, NB0 = MB * MD0;
+  constexpr int T1 = NH == 2 ? (NB1 + kWaves - 1) / kWaves : 1, T0 = (NB0 + kWaves - 1) / kWaves;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  build_frags<H, NH, KB0, true>(smem, mlp, E);
+  __syncthreads();
+  const half8* fW0 = reinterpret_cast<const half8*>(smem);
+  const half8* fW1 = fW0 + C::nW0 * 64;
+  const bf16x8* fW1T = reinterpret_cast<const bf16x8*>(fW1 + C::nW1 * 64);
+  const bf16x8* fW0T = fW1T + C::nW1T * 64;
+  const float* vec = reinterpret_cast<const float*>(fW0T + C::nW0T * 64);
+  uint32_t* tileA = reinterpret_cast<uint32_t*>(smem + C::bwd_lds - C::tile_bytes);
+  uint32_t* tileB = tileA + 64 * kTileP;
+  const int wid = threadIdx.x >> 6;
+  const float inv_beta = 1.f / beta;
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  const int d = 3 + E;
+  const int md_used = (E + 4 + 15) / 16;  // h0 dim blocks holding data
+
+  floatx4 acc1[T1], acc0[T0];
+#pragma unroll
+  for (int t = 0; t < T1; ++t) acc1[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < T0; ++t) acc0[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float db1[MB][4], dwo[MB][4], dbo = 0.f;
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) db1[a][e] = dwo[a][e] = 0.f;
+
+  const int64_t n_steps = ceil_div(n, (int64_t)kBlkPts);
+  for (int64_t step = blockIdx.x; step < n_steps; step += gridDim.x) {
+    const int64_t p = (step * kWaves + wid) * 16 + r;
+    const bool ok = p < n;
+    float h0[KB0][8];
+#pragma unroll
+    for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, E, vec4, p, ok, kb, q, h0[kb]);
+    const float gs = ok ? g_sdf[p] : 0.f;
+    Fwd<H, NH, KB0> F;
+    forward_group<H, NH, KB0, true>(fW0, fW1, vec, h0, beta, inv_beta, F);
+    if (q == 0) dbo += gs;
+    float gz[MB][4];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int u = 16 * mb + 4 * q + e;
+        const float hL = NH == 2 ? F.h2[mb][e] : F.h1[mb][e];
+        const float dL = NH == 2 ? F.d2[mb][e] : F.d1[mb][e];
+        dwo[mb][e] += gs * hL;
+        gz[mb][e] = gs * vec[2 * H + u] * dL;
+      }
+    if (NH == 2) {
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) db1[mb][e] += gz[mb][e];
+      tile_store<MB>(tileA, gz);
+      tile_store<MB>(tileB, F.h1);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < T1; ++t) {
+        const int b = wid + kWaves * t;
+        if (b < NB1) acc1[t] = tile_mma(tileA, b / MB, tileB, b % MB, acc1[t]);
+      }
+      // g_h1 = W1^T gz1 (own group, registers), then through softplus'(z0)
+      bf16x8 gh[KBH], gl[KBH];
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) hid_operand_b<MB>(gz, kb, gh[kb], gl[kb]);
+#pragma unroll
+      for (int mi = 0; mi < MB; ++mi) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const bf16x8 w = fW1T[(mi * KBH + kb) * 64 + l];
+          acc = mma_b(w, gl[kb], acc);
+          acc = mma_b(w, gh[kb], acc);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) gz[mi][e] = acc[e] * F.d1[mi][e];
+      }
+      __syncthreads();  // every wavefront is done reading the tiles
+    }
+    tile_store<MB>(tileA, gz);
+#pragma unroll
+    for (int kb = 0; kb < KB0; ++kb)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        tileB[(32 * kb + 8 * q + j) * kTileP + 16 * wid + r] = pack_pair(h0[kb][j]);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < T0; ++t) {
+      const int b = wid + kWaves * t;
+      if (b < NB0 && (b % MD0) < md_used) acc0[t] = tile_mma(tileA, b / MD0, tileB, b % MD0, acc0[t]);
+    }
+    if (g_enc || g_x) {
+      bf16x8 gh[KBH], gl[KBH];
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) hid_operand_b<MB>(gz, kb, gh[kb], gl[kb]);
+#pragma unroll
+      for (int md = 0; md < MD0; ++md) {
+        if (md >= md_used) break;
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const bf16x8 w = fW0T[(md * KBH + kb) * 64 + l];
+          acc = mma_b(w, gl[kb], acc);
+          acc = mma_b(w, gh[kb], acc);
+        }
+        if (ok) {
+          const int c0 = 16 * md + 4 * q;  // internal dims c0..c0+3 of point p
+          if (vec4 && c0 + 4 <= E) {
+            if (g_enc)
+              *reinterpret_cast<float4*>(g_enc + p * E + c0) = float4{acc[0], acc[1], acc[2], acc[3]};
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int c = c0 + e;
+              if (c < E) {
+                if (g_enc) g_enc[p * E + c] = acc[e];
+              } else if (c < E + 3) {
+                if (g_x) g_x[p * 3 + (c - E)] = acc[e];
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // tiles are rewritten by the next step
+  }
+
+  // the workgroup's partial row: weight blocks from their owners, bias sums
+  // combined over the wavefronts in a fixed order through LDS
+  float* out = partials + (int64_t)blockIdx.x * C::n_params(d);
+#pragma unroll
+  for (int t = 0; t < T0; ++t) {
+    const int b = wid + kWaves * t;
+    if (b >= NB0) continue;
+    const int ob = b / MD0, md = b % MD0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int col = h0_col(16 * md + r, E), o = 16 * ob + 4 * q + e;
+      if (col >= 0) out[o * d + col] = acc0[t][e];
+      else if (col == -2) out[C::gB0(d) + o] = acc0[t][e];
+    }
+  }
+  if (NH == 2) {
+#pragma unroll
+    for (int t = 0; t < T1; ++t) {
+      const int b = wid + kWaves * t;
+      if (b >= NB1) continue;
+      const int ob = b / MB, ib = b % MB;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        out[C::gW1(d) + (16 * ob + 4 * q + e) * H + 16 * ib + r] = acc1[t][e];
+    }
+  }
+  float* red = reinterpret_cast<float*>(tileA);  // [wave][2][H] + [wave] for bo
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = 16 * mb + 4 * q + e;
+      const float s1 = sum_r(db1[mb][e]), so = sum_r(dwo[mb][e]);
+      if (r == 0) {
+        red[(wid * 2 + 0) * H + u] = s1;
+        red[(wid * 2 + 1) * H + u] = so;
+      }
+    }
+  float sbo = sum_r(dbo);
+  sbo += __shfl_xor(sbo, 16, 64);
+  sbo += __shfl_xor(sbo, 32, 64);
+  if (l == 0) red[kWaves * 2 * H + wid] = sbo;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 2 * H + 1; t += blockDim.x) {
+    float v = 0.f;
+    if (t < 2 * H) {
+      for (int w = 0; w < kWaves; ++w) v += red[w * 2 * H + t];
+      const int k = t / H, u = t % H;
+      if (k == 0) { if (NH == 2) out[C::gB1(d) + u] = v; }
+      else out[C::gWo(d) + u] = v;
+    } else {
+      for (int w = 0; w < kWaves; ++w) v += red[kWaves * 2 * H + w];
+      out[C::gBo(d)] = v;
+    }
+  }
+}
+
+// Raises a kernel's dynamic LDS limit above the default 64 KB (gfx950 has
+// 160 KB), once per kernel; a refusal surfaces as a launch error.
+template <typename Kern>
+static void allow_lds(Kern k, size_t bytes) {
+  static const void* done[16];
+  static int n_done = 0;
+  const void* f = reinterpret_cast<const void*>(k);
+  if (bytes <= 65536) return;
+  for (int i = 0; i < n_done; ++i)
+    if (done[i] == f) return;
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (n_done < 16) done[n_done++] = f;
+}
+
+#define SN_MLP16_DISPATCH(H_, NH_, KB0_, ...)                                           \
+  do {                                                                                  \
+    if (H_ == 32 && NH_ == 1 && KB0_ == 1) { constexpr int kH = 32, kNH = 1, kKB = 1; __VA_ARGS__; } \
+    else if (H_ == 32 && NH_ == 1) { constexpr int kH = 32, kNH = 1, kKB = 2; __VA_ARGS__; }          \
+    else if (H_ == 32 && KB0_ == 1) { constexpr int kH = 32, kNH = 2, kKB = 1; __VA_ARGS__; }         \
+    else if (H_ == 32) { constexpr int kH = 32, kNH = 2, kKB = 2; __VA_ARGS__; }                      \
+    else if (NH_ == 1 && KB0_ == 1) { constexpr int kH = 64, kNH = 1, kKB = 1; __VA_ARGS__; }         \
+    else if (NH_ == 1) { constexpr int kH = 64, kNH = 1, kKB = 2; __VA_ARGS__; }                      \
+    else if (KB0_ == 1) { constexpr int kH = 64, kNH = 2, kKB = 1; __VA_ARGS__; }                     \
+    else { constexpr int kH = 64, kNH = 2, kKB = 2; __VA_ARGS__; }                                    \
+  } while (0)
+
+static int check_cfg(const char* who, int E, int H, int NH, int* kb0) {
+  SN_REQUIRE(H == 32 || H == 64, "%s: d_hidden must be 32 or 64 (got %d)", who, H);
+  SN_REQUIRE(NH == 1 || NH == 2, "%s: n_layers must be 1 or 2 (got %d)", who, NH);
+  SN_REQUIRE(E >= 0 && E + 4 <= 64, "%s: encoding width %d must be <= 60", who, E);
+  *kb0 = (E + 4 + 31) / 32;
+  return SN_OK;
+}
+
+// float4 access to rows of E floats
+static int vec4_ok(const float* p, int E) {
+  return (E % 4 == 0) && ((uintptr_t)p % 16 == 0) ? 1 : 0;
+}
+
+constexpr int kFwdBlocksMax = 256 * 4;
+constexpr int kBwdBlocksMax = 256 * 2;
+
+static unsigned blocks_for(int64_t n_cap, int max_blocks) {
+  int64_t b = ceil_div(ceil_div(n_cap > 0 ? n_cap : 1, 16), kWaves);
+  return (unsigned)(b < max_blocks ? b : max_blocks);
+}
+
+}  // namespace m16
+}  // namespace sn
+
+using namespace sn;
+using namespace sn::m16;
+
+extern "C" {
+
+int sn_mlp16_fwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                 const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                 float beta, float* sdf, void* stream) {
+  clear_error();
+  int kb0 = 0;
+  int rc = check_cfg("sn_mlp16_fwd", enc_dims, d_hidden, n_layers, &kb0);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0 && beta > 0.f, "sn_mlp16_fwd: bad n / beta");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && mlp && sdf && (enc || enc_dims == 0), "sn_mlp16_fwd: null pointer");
+  const unsigned grid = blocks_for(n_cap, kFwdBlocksMax);
+  hipStream_t s = as_stream(stream);
+  SN_MLP16_DISPATCH(d_hidden, n_layers, kb0, {
+    constexpr size_t lds = Cfg<kH, kNH, kKB>::fwd_lds;
+    allow_lds(mlp16_fwd_kernel<kH, kNH, kKB>, lds);
+    mlp16_fwd_kernel<kH, kNH, kKB><<<grid, 64 * kWaves, lds, s>>>(
+        x, enc, enc_dims, vec4_ok(enc, enc_dims), n_cap, n_dev, mlp, beta, sdf);
+  });
+  return check_launch("sn_mlp16_fwd");
+}
+
+size_t sn_mlp16_bwd_workspace_bytes(int64_t n_cap, int32_t enc_dims, int32_t d_hidden,
+                                    int32_t n_layers) {
+  const int np = sn_mlp_n_params(3 + enc_dims, d_hidden, n_layers);
+  if (np < 0) return 0;
+  return (size_t)blocks_for(n_cap, kBwdBlocksMax) * np * sizeof(float);
+}
+
+int sn_mlp16_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                 const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                 float beta, const float* g_sdf, float* g_enc, float* g_x, float* g_mlp,
+                 int32_t accumulate, void* ws, size_t ws_bytes, void* stream) {
+  clear_error();
+  int kb0 = 0;
+  int rc = check_cfg("sn_mlp16_bwd", enc_dims, d_hidden, n_layers, &kb0);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0 && beta > 0.f, "sn_mlp16_bwd: bad n / beta");
+  SN_REQUIRE(mlp && g_mlp, "sn_mlp16_bwd: null pointer");
+  const int np = sn_mlp_n_params(3 + enc_dims, d_hidden, n_layers);
+  hipStream_t s = as_stream(stream);
+  if (n_cap == 0) {
+    if (accumulate) return SN_OK;
+    return fill_f32(g_mlp, np, 0.f, s);
+  }
+  SN_REQUIRE(x && g_sdf && (enc || enc_dims == 0), "sn_mlp16_bwd: null input");
+  const size_t need = sn_mlp16_bwd_workspace_bytes(n_cap, enc_dims, d_hidden, n_layers);
+  if (ws_bytes < need) {
+    set_error("sn_mlp16_bwd: workspace %zu < %zu bytes", ws_bytes, need);
+    return SN_ESPACE;
+  }
+  const unsigned grid = blocks_for(n_cap, kBwdBlocksMax);
+  float* part = reinterpret_cast<float*>(ws);
+  SN_MLP16_DISPATCH(d_hidden, n_layers, kb0, {
+    constexpr size_t lds = Cfg<kH, kNH, kKB>::bwd_lds;
+    allow_lds(mlp16_bwd_kernel<kH, kNH, kKB>, lds);
+    mlp16_bwd_kernel<kH, kNH, kKB><<<grid, 64 * kWaves, lds, s>>>(
+        x, enc, enc_dims, vec4_ok(enc, enc_dims) && vec4_ok(g_enc, enc_dims), n_cap, n_dev, mlp,
+        beta, g_sdf, g_enc, g_x, part);
+  });
+  if ((rc = check_launch("sn_mlp16_bwd"))) return rc;
+  return mlp_reduce_rows(part, (int64_t)grid, np, g_mlp, accumulate, s);
+}
+
+}  // extern "C"

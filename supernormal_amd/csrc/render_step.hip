@@ -77,7 +77,11 @@ render_step_kernel(RenderArgs a) {
   const int kc = (Ph / 2) * Pw + Pw / 2;
   const float inv_s = *a.inv_s;
   const float mask_sum = *a.mask_sum;
-  const float n_eik = (float)(*a.n_samples) * (float)P;
+  const int64_t n_s = *a.n_samples;
+  const float n_eik = (float)n_s * (float)P;
+  // ends stored right after the S*P starts when no separate buffer is given
+  const float* sdf_e = a.sdf_e ? a.sdf_e : a.sdf_s + n_s * P;
+  float* g_sdf_e = a.sdf_e ? a.g_sdf_e : a.g_sdf_s + n_s * P;
   const float n_mask = (float)a.n_patches * (float)P;
   const bool ad = a.grad_s != nullptr;
 
@@ -121,7 +125,7 @@ render_step_kernel(RenderArgs a) {
     const int s = base + j;
     f.fs = act ? a.sdf_s[(int64_t)s * P + k] : 0.f;
     int di = act ? a.diff_idx[s] : 0;
-    if (act) f.fe = di >= 0 ? a.sdf_e[(int64_t)di * P + k] : a.sdf_s[(int64_t)(s + 1) * P + k];
+    if (act) f.fe = di >= 0 ? sdf_e[(int64_t)di * P + k] : a.sdf_s[(int64_t)(s + 1) * P + k];
     else f.fe = 0.f;
     const float t0c = act ? a.t0[s] : 0.f, t1c = act ? a.t1[s] : 1.f;
     f.ts = (t0c * dcn) / dkn;
@@ -274,7 +278,7 @@ render_step_kernel(RenderArgs a) {
       // the end value: its own SDF row, or the next sample's start
       const int di = a.diff_idx[s];
       if (di >= 0) {
-        a.g_sdf_e[(int64_t)di * P + k] = g_fe;
+        g_sdf_e[(int64_t)di * P + k] = g_fe;
         if (j + 1 < cnt) a.g_sdf_s[(int64_t)(s + 1) * P + k] = carry;
       } else {
         a.g_sdf_s[(int64_t)(s + 1) * P + k] = carry + g_fe;

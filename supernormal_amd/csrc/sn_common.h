@@ -48,6 +48,10 @@ int fill_f32(float* p, int64_t n, float v, hipStream_t s);
 int scan_flags_dev(const uint8_t* flags, int64_t n_cap, const int32_t* n_dev, int32_t* offsets,
                    int32_t* total, void* ws, size_t ws_bytes, hipStream_t s);
 
+// mlp.hip: out[c] (+)= sum over rows of part[row][c], fixed order
+int mlp_reduce_rows(const float* part, int64_t rows, int cols, float* out, int accumulate,
+                    hipStream_t s);
+
 __device__ __forceinline__ int64_t dev_count(int64_t n_cap, const int32_t* n_dev) {
   if (!n_dev) return n_cap;
   const int64_t v = *n_dev;

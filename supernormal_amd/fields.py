@@ -138,7 +138,8 @@ class SDFNetwork(nn.Module):
             raise ValueError("fused SDF kernels need input_concat, no skips, d_out 1, 1-2 hidden "
                              "layers of width 32 or 64 and 2*bandwidth divisible by the "
                              "features per level")
-        return SDFConfig(self.encoding.grid, self.dims[1], n_hidden, beta=100.0)
+        return SDFConfig(self.encoding.grid, self.dims[1], n_hidden, beta=100.0,
+                         precision=getattr(self, "mlp_precision", "fp32"))
 
     def packed_mlp(self):
         from .sdf import pack_mlp

@@ -76,7 +76,7 @@ class FlatParams:
         self.step_count += 1
         call("sn_adam_step", ptr(self.data), ptr(self.grad), ptr(self.exp_avg),
              ptr(self.exp_avg_sq), self.n, float(lr), None, float(betas[0]), float(betas[1]),
-             float(eps), self.step_count, stream_of(self.data.device))
+             float(eps), self.step_count, None, stream_of(self.data.device))
 
 
 class _RenderStepFn(torch.autograd.Function):

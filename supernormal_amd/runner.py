@@ -80,8 +80,16 @@ class Runner:
         self.occ_update_freq = c.get_int("model.ray_marching.occ_update_freq", 8)
         self.occ_resolution = c.get_int("model.ray_marching.occ_resolution", 128)
         self.max_samples_per_ray = c.get_int("model.ray_marching.max_samples_per_ray", 0)
+        # extensions: MLP precision of the fused kernels ('fp32' | 'fp16') and the
+        # iteration driver ('fused': FusedStep; 'sync_free' / 'graph': SyncFreeStep,
+        # eager or replayed as one HIP graph)
+        self.mlp_precision = c.get("train.mlp_precision", "fp32")
+        self.step_mode = c.get("train.step_mode", "fused")
+        if self.step_mode not in ("fused", "sync_free", "graph"):
+            raise ValueError(f"unknown train.step_mode {self.step_mode!r}")
         self.sdf_network = SDFNetwork(**c["model.sdf_network"],
                                       encoding_config=c["model.encoding"]).to(self.device)
+        self.sdf_network.mlp_precision = self.mlp_precision
         self.deviation_network = SingleVarianceNetwork(**c["model.variance_network"]).to(self.device)
         self.renderer = NeuSRenderer(self.sdf_network, self.deviation_network,
                                      self.gradient_method, occ_resolution=self.occ_resolution,
@@ -94,6 +102,15 @@ class Runner:
                                     self.renderer.occupancy_grid, self.gradient_method,
                                     self.max_samples_per_ray, self.normal_weight,
                                     self.eikonal_weight, self.mask_weight, self.loss_type)
+        self.sf_step = None
+        if fused and self.step_mode != "fused":
+            from .step import SyncFreeStep
+            self.sf_step = SyncFreeStep(
+                self.sdf_network, self.deviation_network, self.renderer.occupancy_grid,
+                self.optimizer, self.dataset, self.batch_size, self.patch_size,
+                self.max_samples_per_ray, self.normal_weight, self.eikonal_weight,
+                self.mask_weight, self.loss_type, precision=self.mlp_precision,
+                device=self.device)
         self.mode = mode
         self.history = []
         if is_continue:
@@ -120,6 +137,12 @@ class Runner:
                                       occ_thre=self.occ_threshold, n=self.occ_update_freq)
         if self.iter_step % self.increase_bindwidth_every == 0:
             self.sdf_network.increase_bandwidth()
+        if self.fused and self.sf_step is not None:
+            losses = self.sf_step.step(self.lr, r.sampling_step_size,
+                                       use_graph=self.step_mode == "graph")
+            self.iter_step += 1
+            self.update_learning_rate()
+            return losses[0] + losses[1] + losses[2]
         o, d, plane_n, v_inv, n_true, mask = self.dataset.gen_random_patches(
             self.batch_size, patch_H=self.patch_size, patch_W=self.patch_size)
         pc = self.patch_size // 2
@@ -227,7 +250,8 @@ class Runner:
 
 
 def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=5000, lr=None,
-                   batch_size=None, max_samples_per_ray=None) -> Conf:
+                   batch_size=None, max_samples_per_ray=None, mlp_precision="fp32",
+                   step_mode="fused") -> Conf:
     """The synthetic training workloads as configs with the reference's keys.
 
     sphere128: sphere r 0.5, 8 views 128^2, hash 8 levels x 2, T 2^15, base 16,
@@ -259,7 +283,8 @@ def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=5000, lr=N
         raise ValueError(workload)
     train.update(learning_rate_alpha=0.05, patch_size=3, warm_up_end=50, loss_type="l2",
                  normal_weight=1.0, eikonal_weight=1.0, mask_weight=1.0,
-                 gradient_method=gradient_method)
+                 gradient_method=gradient_method, mlp_precision=mlp_precision,
+                 step_mode=step_mode)
     if end_iter is not None and workload != "sphere128":
         train["end_iter"] = end_iter
     if lr is not None:

@@ -280,9 +280,11 @@ class Dataset:
         V_inv = self.V_inverse_all[imgx, ys, xs]
         mask = self.masks[imgx, ys, xs].unsqueeze(-1)
         p = torch.stack([xs, ys, torch.ones_like(ys)], -1).float()
-        p = (self.intrinsics_all_inv[imgx, :3, :3] @ p[..., None])[..., 0]
+        # 3x3 products as broadcast multiply-adds: batched GEMMs of this shape
+        # cost far more in launch and tiling than the arithmetic
+        p = (self.intrinsics_all_inv[imgx, :3, :3] * p[..., None, :]).sum(-1)
         d = p / p.norm(dim=-1, keepdim=True)
-        d = (self.pose_all[img, None, None, :3, :3] @ d[..., None])[..., 0]
+        d = (self.pose_all[img, None, None, :3, :3] * d[..., None, :]).sum(-1)
         o = self.pose_all[img, None, None, :3, 3].expand(d.shape)
         plane_n = self.pose_all[img, :3, 2].expand((num_patch, 3))
         return o, d, plane_n, V_inv, normal, mask

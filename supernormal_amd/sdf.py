@@ -25,12 +25,18 @@ from ._lib import call, ptr, stream_of, workspace
 class SDFConfig:
     """Static shape of the field: hash grid + MLP."""
 
-    def __init__(self, grid: dict, d_hidden: int, n_layers: int, beta: float = 100.0):
+    def __init__(self, grid: dict, d_hidden: int, n_layers: int, beta: float = 100.0,
+                 precision: str = "fp32"):
+        if precision not in ("fp32", "fp16"):
+            raise ValueError("precision must be 'fp32' or 'fp16'")
         self.grid = grid
         self.d_hidden = int(d_hidden)
         self.n_layers = int(n_layers)
         self.beta = float(beta)
         self.E = grid["n_levels"] * grid["n_features"]
+        # 'fp16': MLP on the fp16/bf16 matrix cores (sn_mlp16_*) for the sdf
+        # value and its backward; the analytic gradient stays on fp32 kernels
+        self.precision = precision
 
     def hash_args(self, n_active):
         g = self.grid
@@ -53,6 +59,10 @@ def _hash_fwd(cfg, x, params, n_active, n_dev=None):
 def _mlp_fwd(cfg, x, enc, mlp, want_dh0):
     n = x.shape[0]
     sdf = torch.empty(n, dtype=torch.float32, device=x.device)
+    if cfg.precision == "fp16" and not want_dh0:
+        call("sn_mlp16_fwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden,
+             cfg.n_layers, cfg.beta, ptr(sdf), stream_of(x.device))
+        return sdf, None
     dh0 = torch.empty((n, 3 + cfg.E), dtype=torch.float32, device=x.device) if want_dh0 else None
     call("sn_mlp_fwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden, cfg.n_layers,
          cfg.beta, ptr(sdf), ptr(dh0), stream_of(x.device))
@@ -81,12 +91,19 @@ class _SDFFn(torch.autograd.Function):
         g_enc = torch.empty((n, cfg.E), dtype=torch.float32, device=dev)
         g_mlp = torch.empty_like(mlp)
         lib = _lib.lib()
-        ws = workspace(lib.sn_mlp_bwd_workspace_bytes(n, cfg.E, cfg.d_hidden, cfg.n_layers, 0),
-                       dev)
         s = stream_of(dev)
-        call("sn_mlp_bwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden,
-             cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
-             ws.numel(), s)
+        if cfg.precision == "fp16":
+            ws = workspace(lib.sn_mlp16_bwd_workspace_bytes(n, cfg.E, cfg.d_hidden,
+                                                            cfg.n_layers), dev)
+            call("sn_mlp16_bwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden,
+                 cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
+                 ws.numel(), s)
+        else:
+            ws = workspace(lib.sn_mlp_bwd_workspace_bytes(n, cfg.E, cfg.d_hidden, cfg.n_layers,
+                                                          0), dev)
+            call("sn_mlp_bwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden,
+                 cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
+                 ws.numel(), s)
         g_params = torch.zeros_like(params)
         call("sn_hashgrid_bwd", ptr(x), n, ptr(params), *cfg.hash_args(ctx.n_active), ptr(g_enc),
              ptr(g_params), None, s)

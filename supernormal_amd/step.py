@@ -1,0 +1,290 @@
+"""The training iteration with no host round trip, optionally as one HIP graph.
+
+FusedStep (fused.py) reads three data-dependent sizes back to the host per
+iteration, as the reference does (ray_marching.cu:261, ray_marching.py:216-220,
+renderer.py:157).  Here every such size stays on the device: buffers are
+allocated once at capacity (rays x max_samples_per_ray centre samples, and
+x patch size for the patch rays), each kernel launch is sized by the capacity
+and reads the valid count from device memory.  The iteration is then a fixed
+sequence of launches on one stream:
+
+  patch sampling (dataset_loader.py:223-277, device RNG)
+  -> stratified jitter, sn_ray_march_count / scan / _fill (renderer.py:66-79)
+  -> sn_centre_points, SDF forward, sn_alpha_visibility, sn_compact_samples_dev,
+     sn_pack_info_dev (the visibility pre-pass, renderer.py:80-135)
+  -> sn_patch_points (patch expansion, renderer.py:142-159)
+  -> hash-grid + MLP forward on every patch sample (renderer.py:161)
+  -> sn_render_step (weights, normals, losses and their backward)
+  -> MLP backward, hash-grid backward into the flat gradient buffer, weight-
+     norm and variance chain (autograd over the small parameter tensors)
+  -> sn_adam_step with the learning rate and step count read from the device
+
+and can be captured once into a hipGraph (torch.cuda.CUDAGraph) and replayed.
+The occupancy-grid update (every n iterations) and the bandwidth increase run
+between replays; a change of the active hash levels re-captures the graph.
+
+Only the 'dfd' gradient method is supported here (the analytic gradient uses
+FusedStep), and max_samples_per_ray must be set (it bounds the capacity).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+
+
+class SyncFreeStep:
+    def __init__(self, sdf_network, deviation_network, occupancy_grid, flat, dataset,
+                 batch_size, patch_size=3, max_samples_per_ray=128, normal_weight=1.0,
+                 eikonal_weight=1.0, mask_weight=1.0, loss_type="l2", early_stop_eps=1e-8,
+                 precision="fp16", device="cuda"):
+        if max_samples_per_ray <= 0:
+            raise ValueError("SyncFreeStep needs max_samples_per_ray > 0 (it bounds the buffers)")
+        if precision not in ("fp32", "fp16"):
+            raise ValueError("precision must be 'fp32' or 'fp16'")
+        self.net, self.dev_net, self.grid, self.flat = sdf_network, deviation_network, \
+            occupancy_grid, flat
+        self.dataset = dataset
+        self.dev = torch.device(device)
+        self.B, self.Ph, self.Pw = int(batch_size), int(patch_size), int(patch_size)
+        self.P = self.Ph * self.Pw
+        self.msr = int(max_samples_per_ray)
+        self.normal_w, self.eik_w, self.mask_w = float(normal_weight), float(eikonal_weight), \
+            float(mask_weight)
+        self.l1 = loss_type == "l1"
+        self.eps = float(early_stop_eps)
+        self.precision = precision
+        cfg = sdf_network.sdf_config()
+        self.cfg = cfg
+        self.E = cfg.E
+        self._alloc()
+        self.adam_t.fill_(int(flat.step_count))
+        self.graph = None
+        self._graph_levels = None
+
+    # ------------------------------------------------------------------ buffers
+    def _alloc(self):
+        dev, B, P, E = self.dev, self.B, self.P, self.E
+        lib = _lib.lib()
+        f32 = dict(dtype=torch.float32, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        c0 = B * self.msr                       # centre samples
+        c1 = 2 * c0 * P                         # patch points (starts + non-contiguous ends)
+        self.c0, self.c1 = c0, c1
+        Ph, Pw = self.Ph, self.Pw
+        # inputs of one iteration (written by the sampling step)
+        self.rays_o = torch.zeros(B, Ph, Pw, 3, **f32)
+        self.rays_d = torch.zeros(B, Ph, Pw, 3, **f32)
+        self.plane_n = torch.zeros(B, 3, **f32)
+        self.v_inv = torch.zeros(B, Ph, Pw, 3, 3, **f32)
+        self.n_true = torch.zeros(B, Ph, Pw, 3, **f32)
+        self.mask = torch.zeros(B, P, **f32)
+        self.o_c = torch.zeros(B, 3, **f32)
+        self.d_c = torch.zeros(B, 3, **f32)
+        self.t_min = torch.zeros(B, **f32)
+        self.t_max = torch.zeros(B, **f32)
+        self.mask_sum = torch.zeros(1, **f32)
+        # schedule scalars read by the kernels
+        self.step_size = torch.zeros(1, **f32)
+        self.lr = torch.zeros(1, **f32)
+        self.adam_t = torch.zeros(1, **i32)
+        self.inv_s = torch.zeros(1, **f32)
+        # marching
+        self.counts = torch.zeros(B, **i32)
+        self.packed0 = torch.zeros(B, 2, **i32)
+        self.n0 = torch.zeros(1, **i32)
+        self.ri0 = torch.zeros(c0, dtype=torch.int64, device=dev)
+        self.t00 = torch.zeros(c0, **f32)
+        self.t10 = torch.zeros(c0, **f32)
+        # visibility pre-pass
+        self.didx0 = torch.zeros(c0, **i32)
+        self.nd0 = torch.zeros(1, **i32)
+        self.ne0 = torch.zeros(1, **i32)
+        self.pts0 = torch.zeros(2 * c0, 3, **f32)
+        self.enc0 = torch.zeros(2 * c0, E, **f32)
+        self.sdf0 = torch.zeros(2 * c0, **f32)
+        self.keep = torch.zeros(c0, dtype=torch.uint8, device=dev)
+        # visible samples, patch expansion
+        self.ri = torch.zeros(c0, dtype=torch.int64, device=dev)
+        self.t0 = torch.zeros(c0, **f32)
+        self.t1 = torch.zeros(c0, **f32)
+        self.ns = torch.zeros(1, **i32)
+        self.packed = torch.zeros(B, 2, **i32)
+        self.didx = torch.zeros(c0, **i32)
+        self.nd = torch.zeros(1, **i32)
+        self.ne = torch.zeros(1, **i32)
+        self.pts = torch.zeros(self.c1, 3, **f32)
+        self.enc = torch.zeros(self.c1, E, **f32)
+        self.sdf = torch.zeros(self.c1, **f32)
+        self.g_sdf = torch.zeros(self.c1, **f32)
+        self.g_enc = torch.zeros(self.c1, E, **f32)
+        # render step
+        self.losses = torch.zeros(4, **f32)
+        waves = lib.sn_render_step_waves(B, P)
+        self.partials = torch.zeros(4 * waves, **f32)
+        # workspaces
+        nb = lambda n: torch.zeros(max(int(n), 1), dtype=torch.uint8, device=dev)
+        self.ws_scan = nb(lib.sn_scan_workspace_bytes(B))
+        self.ws_step = nb(lib.sn_step_workspace_bytes(c0))
+        self.ws_compact = nb(lib.sn_compact_workspace_bytes(c0))
+        self.ws_pack = nb(lib.sn_pack_info_workspace_bytes(B))
+        cfg = self.cfg
+        if self.precision == "fp16":
+            wsb = lib.sn_mlp16_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers)
+        else:
+            wsb = lib.sn_mlp_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers, 0)
+        self.ws_mlp = nb(wsb)
+        n_mlp = lib.sn_mlp_n_params(3 + E, cfg.d_hidden, cfg.n_layers)
+        self.g_mlp = torch.zeros(n_mlp, **f32)
+
+    # ------------------------------------------------------------- one iteration
+    def _sample(self):
+        """Patch sampling into the static input buffers (device RNG only)."""
+        ds = self.dataset
+        o, d, pn, vi, nt, m = ds.gen_random_patches(self.B, patch_H=self.Ph, patch_W=self.Pw)
+        self._load(o, d, pn, vi, nt, m, stratified=True)
+
+    def _load(self, o, d, pn, vi, nt, m, stratified):
+        ds = self.dataset
+        self.rays_o.copy_(o)
+        self.rays_d.copy_(d)
+        self.plane_n.copy_(pn)
+        self.v_inv.copy_(vi)
+        self.n_true.copy_(nt)
+        mb = (m > 0.5).float() if self.mask_w > 0 else torch.ones_like(m)
+        self.mask.copy_(mb.reshape(self.B, self.P))
+        self.mask_sum.copy_(self.mask.sum().reshape(1) + 1e-5)
+        kc = (self.Ph // 2, self.Pw // 2)
+        self.o_c.copy_(self.rays_o[:, kc[0], kc[1]])
+        self.d_c.copy_(self.rays_d[:, kc[0], kc[1]])
+        near, far = ds.near_far_from_sphere(self.o_c, self.d_c)
+        # stratified start (renderer.py:69-71)
+        if stratified:
+            near = near + torch.rand_like(near) * self.step_size
+        self.t_min.copy_(near)
+        self.t_max.copy_(far)
+
+    def _hash_args(self):
+        return self.cfg.hash_args(self.net.active_levels())
+
+    def _sdf_fwd(self, pts, enc, sdf, cap, n_dev, mlp, s):
+        cfg = self.cfg
+        call("sn_hashgrid_fwd_dev", ptr(pts), cap, ptr(n_dev), ptr(self.net.encoding.params),
+             *self._hash_args(), ptr(enc), s)
+        if self.precision == "fp16":
+            call("sn_mlp16_fwd", ptr(pts), ptr(enc), cfg.E, cap, ptr(n_dev), ptr(mlp),
+                 cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), s)
+        else:
+            call("sn_mlp_fwd", ptr(pts), ptr(enc), cfg.E, cap, ptr(n_dev), ptr(mlp),
+                 cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), None, s)
+
+    def _iteration(self, inputs=None):
+        s = stream_of(self.dev)
+        B, P, c0, c1 = self.B, self.P, self.c0, self.c1
+        cfg = self.cfg
+        grid = self.grid
+        rx, ry, rz = (int(v) for v in grid.binary.shape)
+        self.flat.grad.zero_()
+        if inputs is None:
+            self._sample()
+        elif not isinstance(inputs, str):  # "keep": the buffers hold the inputs already
+            self._load(*inputs, stratified=False)
+        # weight-normed MLP weights and inv_s, differentiable w.r.t. the parameters
+        with torch.enable_grad():
+            mlp = self.net.packed_mlp()
+            inv_s = torch.exp(self.dev_net.variance * 10.0).clip(1e-6, 1e6).reshape(1)
+        self.inv_s.copy_(inv_s.detach())
+        mlp_d = mlp.detach()
+        roi = grid.roi_aabb
+        march = (ptr(self.o_c), ptr(self.d_c), ptr(self.t_min), ptr(self.t_max), B, ptr(roi),
+                 ptr(grid.binary), rx, ry, rz, 1.0, ptr(self.step_size), 0.0, self.msr)
+        call("sn_ray_march_count", *march, ptr(self.counts), s)
+        call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
+             ptr(self.ws_scan), self.ws_scan.numel(), s)
+        call("sn_ray_march_fill", *march, ptr(self.packed0), ptr(self.ri0), ptr(self.t00),
+             ptr(self.t10), s)
+        # visibility pre-pass on the centre rays
+        call("sn_centre_points", ptr(self.ri0), ptr(self.t00), ptr(self.t10), c0, ptr(self.n0),
+             ptr(self.o_c), ptr(self.d_c), 3, ptr(self.didx0), ptr(self.nd0), ptr(self.pts0),
+             ptr(self.ne0), ptr(self.ws_step), self.ws_step.numel(), s)
+        self._sdf_fwd(self.pts0, self.enc0, self.sdf0, 2 * c0, self.ne0, mlp_d, s)
+        call("sn_alpha_visibility", ptr(self.packed0), B, ptr(self.sdf0), ptr(self.didx0), c0,
+             ptr(self.n0), ptr(self.inv_s), self.eps, ptr(self.keep), s)
+        call("sn_compact_samples_dev", ptr(self.keep), c0, ptr(self.n0), ptr(self.ri0),
+             ptr(self.t00), ptr(self.t10), ptr(self.ri), ptr(self.t0), ptr(self.t1),
+             ptr(self.ns), ptr(self.ws_compact), self.ws_compact.numel(), s)
+        call("sn_pack_info_dev", ptr(self.ri), c0, ptr(self.ns), B, ptr(self.packed), None,
+             ptr(self.ws_pack), self.ws_pack.numel(), s)
+        # patch expansion and the SDF on every patch sample
+        call("sn_patch_points", ptr(self.ri), ptr(self.t0), ptr(self.t1), c0, ptr(self.ns),
+             ptr(self.rays_o), ptr(self.rays_d), ptr(self.plane_n), self.Ph, self.Pw,
+             ptr(self.didx), ptr(self.nd), ptr(self.pts), ptr(self.ne), ptr(self.ws_step),
+             self.ws_step.numel(), s)
+        self._sdf_fwd(self.pts, self.enc, self.sdf, c1, self.ne, mlp_d, s)
+        call("sn_render_step", B, self.Ph, self.Pw, ptr(self.rays_o), ptr(self.rays_d),
+             ptr(self.plane_n), ptr(self.v_inv), ptr(self.n_true), ptr(self.mask),
+             ptr(self.packed), ptr(self.t0), ptr(self.t1), ptr(self.didx), ptr(self.sdf), None,
+             None, ptr(self.inv_s), ptr(self.mask_sum), ptr(self.ns), self.normal_w, self.eik_w,
+             self.mask_w, 1 if self.l1 else 0, ptr(self.g_sdf), None, None, ptr(self.losses),
+             ptr(self.partials), None, None, s)
+        # backward: MLP, hash grid (atomics straight into the flat gradient)
+        if self.precision == "fp16":
+            call("sn_mlp16_bwd", ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne),
+                 ptr(mlp_d), cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.g_sdf),
+                 ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
+                 self.ws_mlp.numel(), s)
+        else:
+            call("sn_mlp_bwd", ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne),
+                 ptr(mlp_d), cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.g_sdf),
+                 ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
+                 self.ws_mlp.numel(), s)
+        call("sn_hashgrid_bwd_dev", ptr(self.pts), c1, ptr(self.ne),
+             ptr(self.net.encoding.params), *self._hash_args(), ptr(self.g_enc),
+             ptr(self.net.encoding.params.grad), None, s)
+        # weight norm (packed weights -> weight_v, weight_g, bias) and inv_s -> variance
+        torch.autograd.backward([mlp, inv_s], [self.g_mlp, self.losses[3:4]])
+        # Adam with lr and step count on the device
+        self.adam_t += 1
+        f = self.flat
+        call("sn_adam_step", ptr(f.data), ptr(f.grad), ptr(f.exp_avg), ptr(f.exp_avg_sq), f.n,
+             0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0, ptr(self.adam_t), s)
+
+    # ------------------------------------------------------------------- driver
+    def step(self, lr, step_size, use_graph=True, inputs=None):
+        """One training iteration; lr / step_size for this iteration.  Returns
+        the device tensor of losses (normal, mask, eikonal, dL/dinv_s).
+        `inputs` = (rays_o, rays_d, plane_n, V_inv, normals, mask) of fixed
+        patches with unjittered starts instead of sampling, or "keep" to rerun
+        on the inputs of the previous iteration (eager only)."""
+        self.lr.fill_(float(lr))
+        self.step_size.fill_(float(step_size))
+        if not use_graph or inputs is not None:
+            self._iteration(inputs)
+            self.flat.step_count += 1
+            return self.losses
+        levels = self.net.active_levels()
+        if self.graph is None or self._graph_levels != levels:
+            self._capture()
+            self._graph_levels = levels
+        self.graph.replay()
+        self.flat.step_count += 1
+        return self.losses
+
+    def _capture(self):
+        # one eager warm-up on a side stream (allocator pools, kernel attributes),
+        # then capture; the warm-up's update is undone from saved state
+        f = self.flat
+        saved = [t.clone() for t in (f.data, f.exp_avg, f.exp_avg_sq, self.adam_t)]
+        side = torch.cuda.Stream(device=self.dev)
+        side.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(side):
+            self._iteration()
+        torch.cuda.current_stream(self.dev).wait_stream(side)
+        torch.cuda.synchronize(self.dev)
+        for t, v in zip((f.data, f.exp_avg, f.exp_avg_sq, self.adam_t), saved):
+            t.copy_(v)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._iteration()
+        self.graph = g

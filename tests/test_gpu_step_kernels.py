@@ -117,9 +117,11 @@ def test_alpha_visibility(dev, B, seed):
     didx = torch.as_tensor(_pad(slots, cap - S, -1), device=dev)
     keep = torch.full((cap,), 9, dtype=torch.uint8, device=dev)
     keep[:S] = 0
-    call("sn_alpha_visibility", ptr(torch.as_tensor(packed, device=dev)), B, ptr(sdf_cap),
-         ptr(didx), cap, ptr(n_dev), ptr(torch.tensor([inv_s], device=dev)), 1e-4, ptr(keep),
-         stream_of(dev))
+    # (device tensors held in locals: a temporary's memory can be reused before the launch)
+    packed_d = torch.as_tensor(packed, device=dev)
+    inv_s_d = torch.tensor([inv_s], device=dev)
+    call("sn_alpha_visibility", ptr(packed_d), B, ptr(sdf_cap), ptr(didx), cap, ptr(n_dev),
+         ptr(inv_s_d), 1e-4, ptr(keep), stream_of(dev))
     # fp64 restatement: alpha from (start, end) SDF, keep where T_before >= eps
     fs = sdf[:S].astype(np.float64)
     fe = np.where(slots >= 0, sdf[S + np.maximum(slots, 0)], np.roll(sdf[:S], -1)).astype(np.float64)
@@ -154,9 +156,10 @@ def test_patch_points(dev, B, Ph, Pw, seed):
     pts = torch.zeros(2 * cap * P, 3, device=dev)
     n_diff = torch.zeros(1, dtype=torch.int32, device=dev)
     n_eval = torch.zeros(1, dtype=torch.int32, device=dev)
-    call("sn_patch_points", ptr(rit), ptr(t0t), ptr(t1t), cap, ptr(n_dev), ptr(o.to(dev)),
-         ptr(d.to(dev)), ptr(n.to(dev)), Ph, Pw, ptr(didx), ptr(n_diff), ptr(pts),
-         ptr(n_eval), ptr(ws), ws.numel(), stream_of(dev))
+    od, dd, nd = o.to(dev), d.to(dev), n.to(dev)
+    call("sn_patch_points", ptr(rit), ptr(t0t), ptr(t1t), cap, ptr(n_dev), ptr(od), ptr(dd),
+         ptr(nd), Ph, Pw, ptr(didx), ptr(n_diff), ptr(pts), ptr(n_eval), ptr(ws), ws.numel(),
+         stream_of(dev))
     dm, slots = _diff_slots(ri, t0, t1)
     Nd = int(dm.sum())
     assert int(n_diff) == Nd and int(n_eval) == (S + Nd) * P

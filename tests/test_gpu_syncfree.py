@@ -1,0 +1,89 @@
+"""The sync-free iteration (SyncFreeStep) against FusedStep on the same
+patches, and its HIP-graph replays against eager runs of the same inputs.
+
+FusedStep reads the sample counts back to the host; SyncFreeStep keeps them on
+the device and sizes launches by capacity.  On identical inputs (unjittered
+starts) both must select the same samples (bit-exact counts) and give the same
+loss and gradients up to the order of fp32 atomics (rtol 1e-4).  Graph replays
+are checked step by step against an eager rerun from the same state on the
+replay's own inputs, with eager work and synchronisations between replays.
+"""
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _runner(dev, precision="fp32", step_mode="sync_free"):
+    from supernormal_amd.runner import Runner, synthetic_conf
+    conf = synthetic_conf("sphere128", max_samples_per_ray=32, mlp_precision=precision,
+                          step_mode=step_mode)
+    r = Runner(conf, fused=True, device=dev, seed=3)
+    r.update_learning_rate()
+    # a few ordinary iterations so that the occupancy grid and the field are not trivial
+    for i in range(12):
+        r.train_iteration(i)
+    torch.cuda.synchronize()
+    return r
+
+
+def _state(flat):
+    return [t.clone() for t in (flat.data, flat.exp_avg, flat.exp_avg_sq)]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_sync_free_matches_fused_step(dev, precision):
+    r = _runner(dev, precision)
+    sf, fs, flat = r.sf_step, r.fused_step, r.optimizer
+    g = torch.Generator(device=dev).manual_seed(7)
+    o, d, pn, vi, nt, m = r.dataset.gen_random_patches(r.batch_size, generator=g)
+    near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+    step = 5e-3
+    # FusedStep (host-synchronised sizes, autograd)
+    flat.zero_grad()
+    res = fs.forward_backward(o, d, pn, vi, nt, m, near, far, step, stratified=False)
+    flat.collect()
+    g_ref = flat.grad.clone()
+    loss_ref = float(res.loss)
+    # SyncFreeStep on the same patches (it also takes the Adam step)
+    before = _state(flat)
+    losses = sf.step(r.lr, step, use_graph=False, inputs=(o, d, pn, vi, nt, m))
+    torch.cuda.synchronize()
+    assert int(sf.ns) == res.n_samples
+    assert int(sf.n0) == res.n_marched
+    loss = float(losses[0] + losses[1] + losses[2])
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref) + 1e-7
+    scale = float(g_ref.abs().max())
+    torch.testing.assert_close(flat.grad, g_ref, rtol=1e-4, atol=1e-5 * scale)
+    # the Adam step matches FlatParams.adam on the same gradient
+    after = flat.data.clone()
+    flat.data.copy_(before[0])
+    flat.exp_avg.copy_(before[1])
+    flat.exp_avg_sq.copy_(before[2])
+    flat.step_count -= 1
+    flat.adam(r.lr)
+    torch.testing.assert_close(after, flat.data, rtol=1e-6, atol=1e-7)
+
+
+def test_graph_replays_match_eager(dev):
+    r = _runner(dev, "fp16", "graph")
+    sf, flat = r.sf_step, r.optimizer
+    lr, step = r.lr, 5e-3
+    sf.step(lr, step, use_graph=True)          # capture + first replay
+    for k in range(3):
+        torch.cuda.synchronize()
+        before = _state(flat)
+        t_before = sf.adam_t.clone()
+        losses_g = sf.step(lr, step, use_graph=True).clone()
+        after_g = _state(flat)
+        torch.cuda.synchronize()
+        # eager work between replays, then the same step eagerly on the replay's inputs
+        for t, v in zip((flat.data, flat.exp_avg, flat.exp_avg_sq), before):
+            t.copy_(v)
+        sf.adam_t.copy_(t_before)
+        flat.step_count -= 1
+        losses_e = sf.step(lr, step, use_graph=False, inputs="keep").clone()
+        torch.cuda.synchronize()
+        torch.testing.assert_close(losses_g, losses_e, rtol=1e-4, atol=1e-7)
+        for a, b in zip(after_g, _state(flat)):
+            torch.testing.assert_close(a, b, rtol=1e-4, atol=1e-6)
