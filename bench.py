@@ -110,8 +110,8 @@ def main(argv=None):
     ap.add_argument("--roofline-iters", type=int, default=20)
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs on a "
+              "single GPU and produces no multi-GPU figure.", file=sys.stderr)
         return 2
     from supernormal_amd import _lib
     from supernormal_amd.runner import Runner, synthetic_conf

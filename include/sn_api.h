@@ -131,6 +131,19 @@ int sn_ray_march_fill(const float* rays_o, const float* rays_d, const float* t_m
                       float step_size, const float* step_dev, float cone_angle,
                       int32_t max_per_ray, const int32_t* packed_info, int64_t* ray_indices,
                       float* t_starts, float* t_ends, void* stream);
+/* One-pass form when max_per_ray > 0 bounds every ray: ray i's samples go   */
+/* to slots [i*max_per_ray, i*max_per_ray + num_steps[i]) of the padded      */
+/* arrays; after sn_counts_to_packed_info, sn_march_gather packs them into   */
+/* exactly the (ray_indices, t_starts, t_ends) of count + fill.              */
+int sn_ray_march_padded(const float* rays_o, const float* rays_d, const float* t_min,
+                        const float* t_max, int32_t n_rays, const float* roi_aabb,
+                        const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
+                        float step_size, const float* step_dev, float cone_angle,
+                        int32_t max_per_ray, int32_t* num_steps, float* t_starts_pad,
+                        float* t_ends_pad, void* stream);
+int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_ray,
+                    const float* t_starts_pad, const float* t_ends_pad, int64_t* ray_indices,
+                    float* t_starts, float* t_ends, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Patch accumulation: replaces accumulate_along_rays_patch_based            */
@@ -260,6 +273,9 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
 /* 4 * sn_render_step_waves() floats.                                        */
 /* ------------------------------------------------------------------------ */
 int sn_render_step_waves(int32_t n_patches, int32_t patch_size);
+/* Kernel choice (no counterpart in the reference; for tests): 1 (default)   */
+/* = sample-parallel kernel for 3x3 and 2x4 patches, 0 = ray-parallel.     */
+void sn_render_step_set_impl(int32_t impl);
 int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const float* rays_o,
                    const float* rays_d, const float* plane_n, const float* v_inv,
                    const float* n_true, const float* mask, const int32_t* packed_info,

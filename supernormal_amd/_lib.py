@@ -65,6 +65,10 @@ SIGNATURES = {
     "sn_mlp16_bwd_workspace_bytes": (_sz, [_i64, _i32, _i32, _i32]),
     "sn_mlp16_bwd": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p, _p, _p,
                                _i32, _p, _sz, _p]),
+    "sn_render_step_set_impl": (None, [_i32]),
+    "sn_ray_march_padded": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
+                                      _p, _f32, _i32, _p, _p, _p, _p]),
+    "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz,
                                          _p]),

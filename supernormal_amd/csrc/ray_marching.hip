@@ -60,7 +60,11 @@ __device__ __forceinline__ float distance_to_next_voxel(const float xyz[3], cons
 
 // The roi is a device tensor in the reference's interface; every lane loads
 // the 6 floats (one cached line) instead of a host round trip.
-template <bool kFill>
+// kMode: kCount writes num_steps; kFill writes the packed samples at
+// packed_info's bases; kPadded writes ray i's samples at i * max_per_ray and
+// num_steps in one march (march_gather packs them afterwards).
+enum { kCount = 0, kFill = 1, kPadded = 2 };
+template <int kMode>
 __global__ void __launch_bounds__(64)
 ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restrict__ rays_d,
                      const float* __restrict__ t_min, const float* __restrict__ t_max,
@@ -75,7 +79,7 @@ ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restrict__
 
 using namespace sn;
 
-template <bool kFill>
+template <int kMode>
 __global__ void __launch_bounds__(64)
 sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restrict__ rays_d,
                          const float* __restrict__ t_min, const float* __restrict__ t_max,
@@ -108,15 +112,17 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
   // the step may live in device memory (set per iteration without a host sync)
   const float dt_min = step_dev ? *step_dev : step, dt_max = 1e10f;
   if (!(dt_min > 0.f)) {
-    if (!kFill) num_steps[i] = 0;
+    if (kMode != kFill) num_steps[i] = 0;
     return;
   }
   int64_t base = 0;
   int32_t cap = max_per_ray > 0 ? max_per_ray : 0x7fffffff;
-  if (kFill) {
+  if (kMode == kFill) {
     base = packed_info[2 * i];
     int32_t c = packed_info[2 * i + 1];
     if (c < cap) cap = c;
+  } else if (kMode == kPadded) {
+    base = (int64_t)i * max_per_ray;
   }
   int32_t j = 0;
   float t0 = near;
@@ -129,10 +135,10 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
 #pragma unroll
     for (int d = 0; d < 3; ++d) xyz[d] = o[d] + t_mid * dir[d];
     if (occupied_at(xyz, g, grid)) {
-      if (kFill) {
+      if (kMode != kCount) {
         t_starts[base + j] = t0;
         t_ends[base + j] = t1;
-        ray_indices[base + j] = i;
+        if (kMode == kFill) ray_indices[base + j] = i;
       }
       ++j;
       ++iter;
@@ -153,7 +159,24 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
       t1 = t_mid + dt * 0.5f;
     }
   }
-  if (!kFill) num_steps[i] = j;
+  if (kMode != kFill) num_steps[i] = j;
+}
+
+// Packs the padded per-ray samples at packed_info's bases (exclusive scan of
+// num_steps): the result equals what the fill pass writes.
+__global__ void __launch_bounds__(256)
+march_gather_kernel(const int32_t* __restrict__ packed_info, int32_t n_rays, int32_t slot,
+                    const float* __restrict__ t0p, const float* __restrict__ t1p,
+                    int64_t* __restrict__ ray_indices, float* __restrict__ t_starts,
+                    float* __restrict__ t_ends) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n_rays * slot) return;
+  const int32_t i = (int32_t)(g / slot), j = (int32_t)(g - (int64_t)i * slot);
+  const int32_t base = packed_info[2 * i], cnt = packed_info[2 * i + 1];
+  if (j >= cnt) return;
+  t_starts[base + j] = t0p[g];
+  t_ends[base + j] = t1p[g];
+  ray_indices[base + j] = i;
 }
 
 static int check_march_args(const char* who, const float* o, const float* d, const float* tmin,
@@ -182,7 +205,7 @@ int sn_ray_march_count(const float* rays_o, const float* rays_d, const float* t_
   if (rc) return rc;
   if (n_rays == 0) return SN_OK;
   SN_REQUIRE(num_steps, "sn_ray_march_count: null num_steps");
-  ray_march_kernel_roi<false><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
+  ray_march_kernel_roi<kCount><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
       rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, step_dev,
       cone_angle, max_per_ray, num_steps, nullptr, nullptr, nullptr, nullptr);
   return check_launch("sn_ray_march_count");
@@ -201,10 +224,43 @@ int sn_ray_march_fill(const float* rays_o, const float* rays_d, const float* t_m
   if (n_rays == 0) return SN_OK;
   SN_REQUIRE(packed_info && ray_indices && t_starts && t_ends,
              "sn_ray_march_fill: null output");
-  ray_march_kernel_roi<true><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
+  ray_march_kernel_roi<kFill><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
       rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, step_dev,
       cone_angle, max_per_ray, nullptr, packed_info, ray_indices, t_starts, t_ends);
   return check_launch("sn_ray_march_fill");
+}
+
+int sn_ray_march_padded(const float* rays_o, const float* rays_d, const float* t_min,
+                        const float* t_max, int32_t n_rays, const float* roi_aabb,
+                        const uint8_t* grid, int32_t rx, int32_t ry, int32_t rz,
+                        float step_size, const float* step_dev, float cone_angle,
+                        int32_t max_per_ray, int32_t* num_steps, float* t_starts_pad,
+                        float* t_ends_pad, void* stream) {
+  clear_error();
+  int rc = check_march_args("sn_ray_march_padded", rays_o, rays_d, t_min, t_max, n_rays,
+                            roi_aabb, grid, rx, ry, rz, step_dev ? 1.f : step_size, cone_angle);
+  if (rc) return rc;
+  SN_REQUIRE(max_per_ray > 0, "sn_ray_march_padded: max_per_ray must be > 0");
+  if (n_rays == 0) return SN_OK;
+  SN_REQUIRE(num_steps && t_starts_pad && t_ends_pad, "sn_ray_march_padded: null output");
+  ray_march_kernel_roi<kPadded><<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
+      rays_o, rays_d, t_min, t_max, n_rays, roi_aabb, rx, ry, rz, grid, step_size, step_dev,
+      cone_angle, max_per_ray, num_steps, nullptr, nullptr, t_starts_pad, t_ends_pad);
+  return check_launch("sn_ray_march_padded");
+}
+
+int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_ray,
+                    const float* t_starts_pad, const float* t_ends_pad, int64_t* ray_indices,
+                    float* t_starts, float* t_ends, void* stream) {
+  clear_error();
+  SN_REQUIRE(n_rays >= 0 && max_per_ray > 0, "sn_march_gather: bad sizes");
+  if (n_rays == 0) return SN_OK;
+  SN_REQUIRE(packed_info && t_starts_pad && t_ends_pad && ray_indices && t_starts && t_ends,
+             "sn_march_gather: null pointer");
+  const int64_t total = (int64_t)n_rays * max_per_ray;
+  march_gather_kernel<<<(unsigned)ceil_div(total, 256), 256, 0, as_stream(stream)>>>(
+      packed_info, n_rays, max_per_ray, t_starts_pad, t_ends_pad, ray_indices, t_starts, t_ends);
+  return check_launch("sn_march_gather");
 }
 
 }  // extern "C"

@@ -307,13 +307,327 @@ render_step_kernel(RenderArgs a) {
   }
 }
 
-__global__ void reduce4_kernel(const float* __restrict__ part, int n, float* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= 4) return;
-  float s = 0.f;
-  for (int i = 0; i < n; ++i) s += part[4 * i + c];
-  out[c] = s;
+// ---------------------------------------------------------------------------
+// Sample-parallel form for small patches (Ph x Pw <= 9): one wavefront per
+// patch, lane j = sample j of the patch (in chunks of 64), every ray of the
+// patch in the lane's registers, so the finite differences across the patch
+// are register arithmetic.  Along the samples, the transmittance is an
+// exclusive prefix product and the weight backward's recurrence
+//   Q_j = gw_{j+1} alpha_{j+1} + (1 - alpha_{j+1}) Q_{j+1}
+// a suffix scan of affine maps Q -> b + a Q, both as wavefront scans with
+// lane shuffles; chunks carry T forwards and Q backwards.  Same quantities
+// and outputs as render_step_kernel, summed in another order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
+
+template <int Ph, int Pw, bool AD>
+__global__ void __launch_bounds__(64)
+render_patch_kernel(RenderArgs a) {
+  constexpr int P = Ph * Pw;
+  const int p = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const float inv_s = *a.inv_s;
+  const float mask_sum = *a.mask_sum;
+  const int64_t n_s = *a.n_samples;
+  const float n_eik = (float)n_s * (float)P;
+  const float n_mask = (float)a.n_patches * (float)P;
+  const float* sdf_e = a.sdf_e ? a.sdf_e : a.sdf_s + n_s * P;
+  float* g_sdf_e = a.sdf_e ? a.g_sdf_e : a.g_sdf_s + n_s * P;
+  const int base = a.packed[2 * p], cnt = a.packed[2 * p + 1];
+  constexpr int kc = (Ph / 2) * Pw + Pw / 2;
+
+  // per-ray constants of the patch
+  float o[P][3], d[P][3], dkn[P];
+  const float* nn = a.plane_n + 3 * p;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o[k][c] = a.rays_o[((int64_t)p * P + k) * 3 + c];
+      d[k][c] = a.rays_d[((int64_t)p * P + k) * 3 + c];
+    }
+    dkn[k] = (d[k][0] * nn[0] + d[k][1] * nn[1]) + d[k][2] * nn[2];
+  }
+  const float dcn = (d[kc][0] * nn[0] + d[kc][1] * nn[1]) + d[kc][2] * nn[2];
+
+  // per-sample forward quantities of every ray (lane = sample j)
+  struct Q {
+    float fs[P], fe[P], c[P], e[P], r[P], alpha[P], g[P][3];
+    float gp_coef[P][3];  // dfd: 1/dtd, 1/denx, 1/deny of each ray
+  };
+  auto eval = [&](int j, bool act, Q& q) {
+    const int64_t s = (int64_t)base + j;
+    int di = -1;
+    float t0c = 0.f, t1c = 1.f;
+    if (act) {
+      di = a.diff_idx[s];
+      t0c = a.t0[s];
+      t1c = a.t1[s];
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float fs = 0.f, fe = 0.f;
+      if (act) {
+        fs = a.sdf_s[s * P + k];
+        fe = di >= 0 ? sdf_e[(int64_t)di * P + k] : a.sdf_s[(s + 1) * P + k];
+      }
+      q.fs[k] = fs;
+      q.fe[k] = fe;
+      q.c[k] = sigmoidf_(fs * inv_s);
+      q.e[k] = sigmoidf_(fe * inv_s);
+      q.r[k] = ((q.c[k] - q.e[k]) + 1e-5f) / (q.c[k] + 1e-5f);
+      q.alpha[k] = act ? fminf(fmaxf(q.r[k], 0.f), 1.f) : 0.f;
+    }
+    if (AD) {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q.g[k][c] = act ? a.grad_s[(s * P + k) * 3 + c] : 0.f;
+    } else {
+      float ts[P], te[P], x[P][3];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        ts[k] = (t0c * dcn) / dkn[k];
+        te[k] = (t1c * dcn) / dkn[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[k][c] = o[k][c] + d[k][c] * ts[k];
+      }
+      auto dist = [&](int k1, int k2) {
+        const float dx = x[k2][0] - x[k1][0], dy = x[k2][1] - x[k1][1], dz = x[k2][2] - x[k1][2];
+        return sqrtf(dx * dx + dy * dy + dz * dz);
+      };
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int row = k / Pw, col = k % Pw;
+        const float dtd = te[k] - ts[k];
+        const float df_dt = (q.fe[k] - q.fs[k]) / dtd;
+        float denx, dfx, deny, dfy;
+        if (col == 0) { denx = dist(k, k + 1); dfx = (q.fs[k + 1] - q.fs[k]) / denx; }
+        else if (col == Pw - 1) { denx = dist(k - 1, k); dfx = (q.fs[k] - q.fs[k - 1]) / denx; }
+        else { denx = dist(k - 1, k) + dist(k, k + 1); dfx = (q.fs[k + 1] - q.fs[k - 1]) / denx; }
+        if (row == 0) { deny = dist(k, k + Pw); dfy = (q.fs[k + Pw] - q.fs[k]) / deny; }
+        else if (row == Ph - 1) { deny = dist(k - Pw, k); dfy = (q.fs[k] - q.fs[k - Pw]) / deny; }
+        else { deny = dist(k - Pw, k) + dist(k, k + Pw); dfy = (q.fs[k + Pw] - q.fs[k - Pw]) / deny; }
+        q.gp_coef[k][0] = dtd;
+        q.gp_coef[k][1] = denx;
+        q.gp_coef[k][2] = deny;
+        const float* vi = a.vinv + ((int64_t)p * P + k) * 9;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          q.g[k][c] = act ? (vi[3 * c + 0] * df_dt + vi[3 * c + 1] * dfx + vi[3 * c + 2] * dfy) : 0.f;
+      }
+    }
+  };
+
+  // ---------------- forward: T by prefix products, sums over samples --------
+  float Tin[P], ws[P], cn[P][3], eik[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    Tin[k] = 1.f;
+    ws[k] = 0.f;
+    cn[k][0] = cn[k][1] = cn[k][2] = 0.f;
+    eik[k] = 0.f;
+  }
+  for (int j0 = 0; j0 < cnt; j0 += 64) {
+    const int j = j0 + lane;
+    const bool act = j < cnt;
+    Q q;
+    eval(j, act, q);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      // inclusive prefix product of (1 - alpha), then exclusive
+      float incl = 1.f - q.alpha[k];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const float t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl *= t;
+      }
+      float excl = __shfl_up(incl, 1, 64);
+      if (lane == 0) excl = 1.f;
+      const float T = Tin[k] * excl;
+      const float w = q.alpha[k] * T;
+      if (act) a.g_sdf_s[((int64_t)base + j) * P + k] = T;  // parked for the backward
+      const float nrm = sqrtf(q.g[k][0] * q.g[k][0] + q.g[k][1] * q.g[k][1] + q.g[k][2] * q.g[k][2]);
+      ws[k] += wave_sum(w);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cn[k][c] += wave_sum(w * q.g[k][c]);
+      eik[k] += wave_sum(act ? (nrm - 1.f) * (nrm - 1.f) : 0.f);
+      Tin[k] *= __shfl(incl, 63, 64);
+    }
+  }
+  // ---------------- losses of every ray and their gradients -----------------
+  float g_cn[P][3], g_ws[P];
+  float l_n = 0.f, l_m = 0.f, l_e = 0.f;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const float m = a.mask[(int64_t)p * P + k];
+    float ln = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float nt = a.n_true[((int64_t)p * P + k) * 3 + c];
+      const float err = (cn[k][c] - nt) * m;
+      if (a.l1) {
+        ln += fabsf(err);
+        g_cn[k][c] = a.normal_w * (err > 0.f ? 1.f : (err < 0.f ? -1.f : 0.f)) * m / mask_sum;
+      } else {
+        ln += err * err;
+        g_cn[k][c] = a.normal_w * 2.f * err * m / mask_sum;
+      }
+    }
+    l_n += ln / mask_sum;
+    const float lo = 1e-5f, hi = 1.f - 1e-5f;
+    const float xw = fminf(fmaxf(ws[k], lo), hi);
+    l_m += -(m * fmaxf(logf(xw), -100.f) + (1.f - m) * fmaxf(logf(1.f - xw), -100.f)) / n_mask;
+    const bool pass = ws[k] >= lo && ws[k] <= hi;
+    g_ws[k] = pass ? a.mask_w * (xw - m) / fmaxf((1.f - xw) * xw, 1e-12f) / n_mask : 0.f;
+    l_e += eik[k] / n_eik;
+    if (lane == 0) {
+      if (a.weight_sum) a.weight_sum[(int64_t)p * P + k] = ws[k];
+      if (a.comp_normal)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.comp_normal[((int64_t)p * P + k) * 3 + c] = cn[k][c];
+    }
+  }
+  // ---------------- backward, chunks in reverse ------------------------------
+  float Qin[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) Qin[k] = 0.f;
+  float g_inv = 0.f;
+  const int n_chunks = (cnt + 63) / 64;
+  for (int ch = n_chunks - 1; ch >= 0; --ch) {
+    const int j = ch * 64 + lane;
+    const bool act = j < cnt;
+    const int64_t s = (int64_t)base + j;
+    Q q;
+    eval(j, act, q);
+    const int di = act ? a.diff_idx[s] : 0;
+    float G[P];  // dL/dfs of every ray of this sample (own terms + neighbours')
+    float GE[P]; // dL/dfe
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const float Tj = act ? a.g_sdf_s[s * P + k] : 0.f;
+      const float w = q.alpha[k] * Tj;
+      const float nrm = sqrtf(q.g[k][0] * q.g[k][0] + q.g[k][1] * q.g[k][1] + q.g[k][2] * q.g[k][2]);
+      const float ek = nrm > 0.f ? a.eik_w * 2.f * (nrm - 1.f) / n_eik / nrm : 0.f;
+      float gg[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gg[c] = act ? (w * g_cn[k][c] + ek * q.g[k][c]) : 0.f;
+      const float gw = act ? (g_ws[k] + g_cn[k][0] * q.g[k][0] + g_cn[k][1] * q.g[k][1] +
+                              g_cn[k][2] * q.g[k][2]) : 0.f;
+      // suffix scan of the maps M_j(Q) = gw_j alpha_j + (1 - alpha_j) Q
+      float ma = act ? 1.f - q.alpha[k] : 1.f, mb = act ? gw * q.alpha[k] : 0.f;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const float na = __shfl_down(ma, off, 64), nb = __shfl_down(mb, off, 64);
+        if (lane + off < 64) {
+          mb = mb + ma * nb;
+          ma = ma * na;
+        }
+      }
+      // Q_j = S_{j+1}(Qin) (lane 63: Qin)
+      float sa = __shfl_down(ma, 1, 64), sb = __shfl_down(mb, 1, 64);
+      if (lane == 63) { sa = 1.f; sb = 0.f; }
+      const float Qj = sb + sa * Qin[k];
+      const float Qnext = __shfl(mb + ma * Qin[k], 0, 64);  // S_{first}(Qin): carried back
+      const float g_alpha = act ? Tj * (gw - Qj) : 0.f;
+      const float gr = (q.r[k] >= 0.f && q.r[k] <= 1.f) ? g_alpha : 0.f;
+      const float den = q.c[k] + 1e-5f;
+      const float g_c = gr * q.e[k] / (den * den);
+      const float g_e = -gr / den;
+      G[k] = g_c * inv_s * q.c[k] * (1.f - q.c[k]);
+      GE[k] = g_e * inv_s * q.e[k] * (1.f - q.e[k]);
+      g_inv += act ? (g_c * q.fs[k] * q.c[k] * (1.f - q.c[k]) + g_e * q.fe[k] * q.e[k] * (1.f - q.e[k])) : 0.f;
+      Qin[k] = Qnext;
+      if (AD) {
+        if (act)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a.g_grad_s[(s * P + k) * 3 + c] = gg[c];
+      } else {
+        const float* vi = a.vinv + ((int64_t)p * P + k) * 9;
+        // grad = V^-1 proj  =>  dL/dproj = V^-T dL/dgrad; kept in gg
+        const float gp0 = vi[0] * gg[0] + vi[3] * gg[1] + vi[6] * gg[2];
+        const float gp1 = vi[1] * gg[0] + vi[4] * gg[1] + vi[7] * gg[2];
+        const float gp2 = vi[2] * gg[0] + vi[5] * gg[1] + vi[8] * gg[2];
+        gg[0] = gp0 / q.gp_coef[k][0];
+        gg[1] = act ? gp1 / q.gp_coef[k][1] : 0.f;
+        gg[2] = act ? gp2 / q.gp_coef[k][2] : 0.f;
+        q.gp_coef[k][0] = gg[0];
+        q.gp_coef[k][1] = gg[1];
+        q.gp_coef[k][2] = gg[2];
+      }
+    }
+    if (!AD) {
+      // df_dt = (fe - fs) / dtd; finite differences across the patch
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int row = k / Pw, col = k % Pw;
+        GE[k] += q.gp_coef[k][0];
+        G[k] -= q.gp_coef[k][0];
+        const float ax = q.gp_coef[k][1], ay = q.gp_coef[k][2];
+        if (col == 0) { G[k + 1] += ax; G[k] -= ax; }
+        else if (col == Pw - 1) { G[k] += ax; G[k - 1] -= ax; }
+        else { G[k + 1] += ax; G[k - 1] -= ax; }
+        if (row == 0) { G[k + Pw] += ay; G[k] -= ay; }
+        else if (row == Ph - 1) { G[k] += ay; G[k - Pw] -= ay; }
+        else { G[k + Pw] += ay; G[k - Pw] -= ay; }
+      }
+    }
+    // ends: own SDF row, or the next sample's start (added to that start)
+    const bool to_next = act && di < 0;
+    const int prev_to_next = __shfl_up(to_next ? 1 : 0, 1, 64);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const float from_prev = __shfl_up(GE[k], 1, 64);
+      float gs = G[k];
+      if (lane > 0 && prev_to_next) gs += from_prev;
+      if (act) {
+        // the chunk's first sample may also receive from the previous chunk,
+        // whose lane 63 adds its share later (read-modify-write below)
+        a.g_sdf_s[s * P + k] = gs;
+        if (di >= 0) g_sdf_e[(int64_t)di * P + k] = GE[k];
+      }
+    }
+    if (lane == 63 && to_next && j + 1 < cnt) {
+      // sample j + 1 is the next chunk's first sample, already written
+#pragma unroll
+      for (int k = 0; k < P; ++k) a.g_sdf_s[(s + 1) * P + k] += GE[k];
+    }
+  }
+  // per-patch partials: normal, mask, eikonal, dL/dinv_s
+  const float v3 = wave_sum(g_inv);
+  if (lane == 0) {
+    float* o4 = a.partials + 4 * (int64_t)p;
+    o4[0] = a.normal_w * l_n;
+    o4[1] = a.mask_w * l_m;
+    o4[2] = a.eik_w * l_e;
+    o4[3] = v3;
+  }
+}
+
+// Fixed-order sum of n rows of 4 partials: 256 lanes stride the rows, then the
+// lanes' sums are added in lane order.
+__global__ void __launch_bounds__(256)
+reduce4_wide_kernel(const float* __restrict__ part, int n, float* __restrict__ out) {
+  __shared__ float sh[256][4];
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int i = threadIdx.x; i < n; i += 256)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[c] += part[4 * (int64_t)i + c];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[threadIdx.x][c] = s[c];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    float t = 0.f;
+    for (int i = 0; i < 256; ++i) t += sh[i][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+static int g_render_impl = 1;  // 1: sample-parallel kernel where the patch shape allows
 
 }  // namespace sn
 
@@ -323,8 +637,10 @@ extern "C" {
 
 int sn_render_step_waves(int32_t n_patches, int32_t patch_size) {
   if (patch_size <= 0 || patch_size > 64 || n_patches < 0) return -1;
-  return (int)ceil_div(n_patches, 64 / patch_size);
+  return n_patches;  // one partial row per patch at most (sample-parallel kernel)
 }
+
+void sn_render_step_set_impl(int32_t impl) { g_render_impl = impl; }
 
 int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const float* rays_o,
                    const float* rays_d, const float* plane_n, const float* v_inv,
@@ -379,10 +695,21 @@ int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const fl
   a.partials = partials;
   a.weight_sum = weight_sum;
   a.comp_normal = comp_normal;
-  const int waves = sn_render_step_waves(n_patches, P);
   hipStream_t s = as_stream(stream);
-  render_step_kernel<<<waves, 64, 0, s>>>(a);
-  reduce4_kernel<<<1, 64, 0, s>>>(partials, waves, losses);
+  const bool ad = grad_starts != nullptr;
+  if (g_render_impl == 1 && patch_h == 3 && patch_w == 3) {
+    if (ad) render_patch_kernel<3, 3, true><<<n_patches, 64, 0, s>>>(a);
+    else render_patch_kernel<3, 3, false><<<n_patches, 64, 0, s>>>(a);
+    reduce4_wide_kernel<<<1, 256, 0, s>>>(partials, n_patches, losses);
+  } else if (g_render_impl == 1 && patch_h == 2 && patch_w == 4) {
+    if (ad) render_patch_kernel<2, 4, true><<<n_patches, 64, 0, s>>>(a);
+    else render_patch_kernel<2, 4, false><<<n_patches, 64, 0, s>>>(a);
+    reduce4_wide_kernel<<<1, 256, 0, s>>>(partials, n_patches, losses);
+  } else {
+    const int waves = (int)ceil_div(n_patches, 64 / P);
+    render_step_kernel<<<waves, 64, 0, s>>>(a);
+    reduce4_wide_kernel<<<1, 256, 0, s>>>(partials, waves, losses);
+  }
   return check_launch("sn_render_step");
 }
 

@@ -313,6 +313,9 @@ class OccupancyGrid(nn.Module):
         self.register_buffer("occs", torch.zeros(self.num_cells))
         self._res = tuple(int(r) for r in resolution.tolist())
         self._all = None
+        # extension: a torch.Generator for the update's cell sampling and jitter
+        # (None = torch's default generator, the reference's behaviour)
+        self.generator = None
 
     @property
     def roi_aabb(self) -> Tensor:
@@ -348,10 +351,10 @@ class OccupancyGrid(nn.Module):
 
     @torch.no_grad()
     def _sample_uniform_and_occupied_cells(self, n: int) -> Tensor:
-        uniform = torch.randint(self.num_cells, (n,), device=self.device)
+        uniform = torch.randint(self.num_cells, (n,), device=self.device, generator=self.generator)
         occupied = self._occupied_cells()
         if n < len(occupied):
-            sel = torch.randint(len(occupied), (n,), device=self.device)
+            sel = torch.randint(len(occupied), (n,), device=self.device, generator=self.generator)
             occupied = occupied[sel]
         return torch.cat([uniform, occupied], dim=0)
 
@@ -365,7 +368,7 @@ class OccupancyGrid(nn.Module):
         else:
             indices = self._sample_uniform_and_occupied_cells(self.num_cells // 4)
         n = indices.numel()
-        jitter = torch.rand((n, 3), device=dev, dtype=torch.float32)
+        jitter = torch.rand((n, 3), device=dev, dtype=torch.float32, generator=self.generator)
         x = torch.empty((n, 3), device=dev, dtype=torch.float32)
         rx, ry, rz = self._res
         s = stream_of(dev)

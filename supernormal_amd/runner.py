@@ -45,7 +45,7 @@ from .renderer import NeuSRenderer
 
 class Runner:
     def __init__(self, conf_text, mode="train", is_continue=False, datadir=None, fused=True,
-                 dataset=None, device="cuda", base_exp_dir=None, seed=0):
+                 dataset=None, device="cuda", base_exp_dir=None, seed=0, rng_seed=None):
         self.device = torch.device(device)
         self.conf_text = conf_text if isinstance(conf_text, str) else None
         self.conf = conf_text if isinstance(conf_text, dict) else parse(conf_text)
@@ -110,7 +110,12 @@ class Runner:
                 self.optimizer, self.dataset, self.batch_size, self.patch_size,
                 self.max_samples_per_ray, self.normal_weight, self.eikonal_weight,
                 self.mask_weight, self.loss_type, precision=self.mlp_precision,
-                device=self.device)
+                device=self.device, seed=rng_seed)
+        if rng_seed is not None:
+            # patch sampling and occupancy sampling from generators of this run only
+            g = torch.Generator(device=self.device)
+            g.manual_seed(int(rng_seed) + 1)
+            self.renderer.occupancy_grid.generator = g
         self.mode = mode
         self.history = []
         if is_continue:

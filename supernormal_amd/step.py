@@ -9,7 +9,8 @@ and reads the valid count from device memory.  The iteration is then a fixed
 sequence of launches on one stream:
 
   patch sampling (dataset_loader.py:223-277, device RNG)
-  -> stratified jitter, sn_ray_march_count / scan / _fill (renderer.py:66-79)
+  -> stratified jitter, sn_ray_march_padded / scan / sn_march_gather (one
+     march per ray instead of count + fill, renderer.py:66-79)
   -> sn_centre_points, SDF forward, sn_alpha_visibility, sn_compact_samples_dev,
      sn_pack_info_dev (the visibility pre-pass, renderer.py:80-135)
   -> sn_patch_points (patch expansion, renderer.py:142-159)
@@ -38,7 +39,7 @@ class SyncFreeStep:
     def __init__(self, sdf_network, deviation_network, occupancy_grid, flat, dataset,
                  batch_size, patch_size=3, max_samples_per_ray=128, normal_weight=1.0,
                  eikonal_weight=1.0, mask_weight=1.0, loss_type="l2", early_stop_eps=1e-8,
-                 precision="fp16", device="cuda"):
+                 precision="fp16", device="cuda", seed=None):
         if max_samples_per_ray <= 0:
             raise ValueError("SyncFreeStep needs max_samples_per_ray > 0 (it bounds the buffers)")
         if precision not in ("fp32", "fp16"):
@@ -62,6 +63,12 @@ class SyncFreeStep:
         self.adam_t.fill_(int(flat.step_count))
         self.graph = None
         self._graph_levels = None
+        # a generator of its own (seeded) makes the sampled patches independent
+        # of other work on the device, e.g. other jobs' steps in between
+        self.gen = None
+        if seed is not None:
+            self.gen = torch.Generator(device=self.dev)
+            self.gen.manual_seed(int(seed))
 
     # ------------------------------------------------------------------ buffers
     def _alloc(self):
@@ -97,6 +104,8 @@ class SyncFreeStep:
         self.ri0 = torch.zeros(c0, dtype=torch.int64, device=dev)
         self.t00 = torch.zeros(c0, **f32)
         self.t10 = torch.zeros(c0, **f32)
+        self.t0_pad = torch.zeros(c0, **f32)   # ray i's samples at i * max_samples_per_ray
+        self.t1_pad = torch.zeros(c0, **f32)
         # visibility pre-pass
         self.didx0 = torch.zeros(c0, **i32)
         self.nd0 = torch.zeros(1, **i32)
@@ -142,7 +151,8 @@ class SyncFreeStep:
     def _sample(self):
         """Patch sampling into the static input buffers (device RNG only)."""
         ds = self.dataset
-        o, d, pn, vi, nt, m = ds.gen_random_patches(self.B, patch_H=self.Ph, patch_W=self.Pw)
+        o, d, pn, vi, nt, m = ds.gen_random_patches(self.B, patch_H=self.Ph, patch_W=self.Pw,
+                                                     generator=self.gen)
         self._load(o, d, pn, vi, nt, m, stratified=True)
 
     def _load(self, o, d, pn, vi, nt, m, stratified):
@@ -161,7 +171,8 @@ class SyncFreeStep:
         near, far = ds.near_far_from_sphere(self.o_c, self.d_c)
         # stratified start (renderer.py:69-71)
         if stratified:
-            near = near + torch.rand_like(near) * self.step_size
+            u = torch.rand(near.shape, generator=self.gen, device=near.device, dtype=near.dtype)
+            near = near + u * self.step_size
         self.t_min.copy_(near)
         self.t_max.copy_(far)
 
@@ -197,13 +208,15 @@ class SyncFreeStep:
         self.inv_s.copy_(inv_s.detach())
         mlp_d = mlp.detach()
         roi = grid.roi_aabb
-        march = (ptr(self.o_c), ptr(self.d_c), ptr(self.t_min), ptr(self.t_max), B, ptr(roi),
-                 ptr(grid.binary), rx, ry, rz, 1.0, ptr(self.step_size), 0.0, self.msr)
-        call("sn_ray_march_count", *march, ptr(self.counts), s)
+        # one march into per-ray slots, then packed (= count + fill of the reference)
+        call("sn_ray_march_padded", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
+             ptr(self.t_max), B, ptr(roi), ptr(grid.binary), rx, ry, rz, 1.0,
+             ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
+             ptr(self.t1_pad), s)
         call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
              ptr(self.ws_scan), self.ws_scan.numel(), s)
-        call("sn_ray_march_fill", *march, ptr(self.packed0), ptr(self.ri0), ptr(self.t00),
-             ptr(self.t10), s)
+        call("sn_march_gather", ptr(self.packed0), B, self.msr, ptr(self.t0_pad),
+             ptr(self.t1_pad), ptr(self.ri0), ptr(self.t00), ptr(self.t10), s)
         # visibility pre-pass on the centre rays
         call("sn_centre_points", ptr(self.ri0), ptr(self.t00), ptr(self.t10), c0, ptr(self.n0),
              ptr(self.o_c), ptr(self.d_c), 3, ptr(self.didx0), ptr(self.nd0), ptr(self.pts0),
@@ -276,6 +289,7 @@ class SyncFreeStep:
         # then capture; the warm-up's update is undone from saved state
         f = self.flat
         saved = [t.clone() for t in (f.data, f.exp_avg, f.exp_avg_sq, self.adam_t)]
+        gen_state = self.gen.get_state() if self.gen is not None else None
         side = torch.cuda.Stream(device=self.dev)
         side.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(side):
@@ -284,7 +298,11 @@ class SyncFreeStep:
         torch.cuda.synchronize(self.dev)
         for t, v in zip((f.data, f.exp_avg, f.exp_avg_sq, self.adam_t), saved):
             t.copy_(v)
+        if gen_state is not None:
+            self.gen.set_state(gen_state)
         g = torch.cuda.CUDAGraph()
+        if self.gen is not None:
+            g.register_generator_state(self.gen)
         with torch.cuda.graph(g):
             self._iteration()
         self.graph = g

@@ -207,6 +207,45 @@ def test_ray_marching_bitexact(dev, res, step, cap):
     np.testing.assert_array_equal(t1.cpu().numpy(), t1_o)
 
 
+@pytest.mark.parametrize("res,step,cap", [(128, 1e-3, 128), (64, 2e-3, 128), (32, 5e-3, 7)])
+def test_ray_marching_padded_bitexact(dev, res, step, cap):
+    """One-pass padded march + gather == the oracle's count + fill, with the
+    step size read from device memory."""
+    from supernormal_amd._lib import call, lib, ptr, stream_of
+    rng = np.random.default_rng(res + cap + 1)
+    n = 2048
+    o, d, near, far = _cam_rays(rng, n)
+    near = near + rng.random(n).astype(np.float32) * np.float32(step)
+    grid = _scene_grid(res)
+    roi = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    od, dd, nd, fd, rd = (_t(a, dev) for a in (o, d, near, far, roi))
+    gd = _t(grid.astype(np.uint8), dev)
+    step_d = torch.tensor([step], dtype=torch.float32, device=dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    t0p = torch.full((n * cap,), -1.0, device=dev)
+    t1p = torch.full((n * cap,), -1.0, device=dev)
+    st = stream_of(dev)
+    call("sn_ray_march_padded", ptr(od), ptr(dd), ptr(nd), ptr(fd), n, ptr(rd), ptr(gd), res,
+         res, res, 1.0, ptr(step_d), 0.0, cap, ptr(counts), ptr(t0p), ptr(t1p), st)
+    packed = torch.empty(n, 2, dtype=torch.int32, device=dev)
+    total = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = torch.empty(lib().sn_scan_workspace_bytes(n), dtype=torch.uint8, device=dev)
+    call("sn_counts_to_packed_info", ptr(counts), n, ptr(packed), ptr(total), ptr(ws),
+         ws.numel(), st)
+    ri = torch.empty(n * cap, dtype=torch.int64, device=dev)
+    t0 = torch.empty(n * cap, device=dev)
+    t1 = torch.empty(n * cap, device=dev)
+    call("sn_march_gather", ptr(packed), n, cap, ptr(t0p), ptr(t1p), ptr(ri), ptr(t0), ptr(t1), st)
+    m = int(total)
+    pi_o, ri_o, t0_o, t1_o = c_oracle.ray_marching(o, d, near, far, roi, grid, step,
+                                                   max_per_ray=cap)
+    assert m == ri_o.size and m > 1000
+    np.testing.assert_array_equal(packed.cpu().numpy(), pi_o)
+    np.testing.assert_array_equal(ri[:m].cpu().numpy(), ri_o)
+    np.testing.assert_array_equal(t0[:m].cpu().numpy(), t0_o[:, 0] if t0_o.ndim == 2 else t0_o)
+    np.testing.assert_array_equal(t1[:m].cpu().numpy(), t1_o[:, 0] if t1_o.ndim == 2 else t1_o)
+
+
 def test_ray_marching_visibility_filter(dev):
     """ray_marching with alpha_fn: keep mask and compaction equal the oracle's."""
     from supernormal_amd.nerfacc import ray_marching, OccupancyGrid

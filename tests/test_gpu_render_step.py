@@ -76,14 +76,29 @@ def _sdf_values(sc, g):
     return sdf_s.float().double(), sdf_e.float().double(), dm
 
 
-@pytest.mark.parametrize("method,loss_type,Ph,Pw", [("dfd", "l2", 3, 3), ("ad", "l2", 3, 3),
-                                                    ("dfd", "l1", 3, 3), ("dfd", "l2", 5, 5),
-                                                    ("dfd", "l2", 2, 4)])
-def test_render_step_vs_fp64(method, loss_type, Ph, Pw, dev):
+@pytest.mark.parametrize("impl", [1, 0])
+@pytest.mark.parametrize("method,loss_type,Ph,Pw,max_cnt", [
+    ("dfd", "l2", 3, 3, 40), ("ad", "l2", 3, 3, 40), ("dfd", "l1", 3, 3, 40),
+    ("dfd", "l2", 5, 5, 40), ("dfd", "l2", 2, 4, 40), ("dfd", "l2", 3, 3, 150),
+    ("ad", "l1", 3, 3, 150), ("dfd", "l2", 2, 4, 130)])
+def test_render_step_vs_fp64(method, loss_type, Ph, Pw, max_cnt, impl, dev):
+    """impl 1: sample-parallel kernel (3x3 and 2x4 patches; wavefront scans,
+    several 64-sample chunks when max_cnt > 64); impl 0: ray-parallel kernel."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    _lib.lib().sn_render_step_set_impl(impl)
+    try:
+        _run_render_step(method, loss_type, Ph, Pw, max_cnt, dev)
+    finally:
+        _lib.lib().sn_render_step_set_impl(1)
+
+
+def _run_render_step(method, loss_type, Ph, Pw, max_cnt, dev):
     from supernormal_amd import _lib
     from supernormal_amd._lib import call, ptr, stream_of
     B = 300
-    sc = _scene(B, Ph, Pw, seed=Ph * 10 + Pw + (1 if method == "ad" else 0))
+    sc = _scene(B, Ph, Pw, seed=Ph * 10 + Pw + (1 if method == "ad" else 0) + max_cnt,
+                max_cnt=max_cnt)
     g = torch.Generator().manual_seed(99)
     sdf_s, sdf_e, dm = _sdf_values(sc, g)
     S, Nd, P = sc["ri"].shape[0], sdf_e.shape[0], Ph * Pw
