@@ -303,6 +303,20 @@ int sn_mlp16_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_c
                  int32_t accumulate, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Surface extraction: replaces PyMCubes' marching cubes in extract_geometry */
+/* (renderer.py:27-34) with marching tetrahedra (6 per cube around the main  */
+/* diagonal).  values [rx,ry,rz] (x-major); inside = value < iso.  Pass 1:   */
+/* counts[cube] = triangles; sn_counts_to_packed_info scans them; pass 2     */
+/* writes 9 floats (3 world vertices) per triangle at each cube's base,      */
+/* oriented inside -> outside; world = bound_min + grid * (max-min)/(r-1).   */
+/* ------------------------------------------------------------------------ */
+int sn_mt_count(const float* values, int32_t rx, int32_t ry, int32_t rz, float iso,
+                int32_t* counts, void* stream);
+int sn_mt_emit(const float* values, int32_t rx, int32_t ry, int32_t rz, float iso,
+               const float* bound_min, const float* bound_max, const int32_t* packed_info,
+               float* triangles, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
 /* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
 /* p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps).  `lr` is read from   */

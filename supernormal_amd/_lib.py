@@ -69,6 +69,8 @@ SIGNATURES = {
     "sn_ray_march_padded": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
                                       _p, _f32, _i32, _p, _p, _p, _p]),
     "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
+    "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
+    "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz,
                                          _p]),

@@ -135,78 +135,63 @@ def extract_fields(bound_min, bound_max, resolution, query_func, chunk=64, devic
     return u
 
 
-# Marching tetrahedra: each grid cube is split into 6 tetrahedra along the
-# main diagonal; each tetrahedron yields 0, 1 or 2 triangles.  (The
-# reference uses PyMCubes' marching cubes, which is not installed; the
-# surfaces agree up to the triangulation inside each cube.)
-_CUBE_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
-                          [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
-_TETS = np.array([[0, 5, 1, 6], [0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6],
-                  [0, 4, 5, 6]])
+def extract_fields_device(bound_min, bound_max, resolution, query_func, device, slab=32):
+    """Field values on a resolution^3 grid, kept on the device (renderer.py:9-24:
+    torch.linspace axes, x-major grid)."""
+    r = int(resolution)
+    axes = [torch.linspace(float(bound_min[i]), float(bound_max[i]), r, device=device)
+            for i in range(3)]
+    u = torch.empty((r, r, r), dtype=torch.float32, device=device)
+    yy, zz = torch.meshgrid(axes[1], axes[2], indexing="ij")
+    with torch.no_grad():
+        for x0 in range(0, r, slab):
+            xs = axes[0][x0:x0 + slab]
+            pts = torch.stack([xs[:, None, None].expand(-1, r, r), yy[None].expand(len(xs), -1, -1),
+                               zz[None].expand(len(xs), -1, -1)], -1).reshape(-1, 3)
+            u[x0:x0 + len(xs)] = query_func(pts.contiguous()).reshape(len(xs), r, r)
+    return u
 
 
-def marching_tetrahedra(u, threshold=0.0):
-    """Triangle mesh of {u = threshold}: (vertices [V,3] in grid units, faces [F,3])."""
-    R = np.array(u.shape)
-    idx = np.stack(np.meshgrid(*[np.arange(r - 1) for r in R], indexing="ij"), -1).reshape(-1, 3)
-    vals = np.stack([u[idx[:, 0] + c[0], idx[:, 1] + c[1], idx[:, 2] + c[2]] for c in _CUBE_CORNERS],
-                    -1) - threshold
-    inside = vals < 0
-    any_cross = inside.any(1) & (~inside).any(1)
-    idx, vals = idx[any_cross], vals[any_cross]
-    tris = []
-    for tet in _TETS:
-        tv = vals[:, tet]
-        tp = (idx[:, None, :] + _CUBE_CORNERS[tet][None]).astype(np.float64)
-        neg = tv < 0
-        code = neg.astype(np.int64) @ np.array([1, 2, 4, 8])
-        # direction from the inside corners to the outside corners (orientation)
-        n_in = neg.sum(1, keepdims=True)
-        out_dir = (tp * (~neg)[..., None]).sum(1) / np.maximum(4 - n_in, 1) - \
-            (tp * neg[..., None]).sum(1) / np.maximum(n_in, 1)
-
-        def edge(i, j, sel):
-            a, b = tv[sel, i], tv[sel, j]
-            t = a / (a - b)
-            return tp[sel, i] + (tp[sel, j] - tp[sel, i]) * t[:, None]
-
-        for lone in range(4):  # one corner on its own side: one triangle
-            others = [k for k in range(4) if k != lone]
-            for c in (1 << lone, 15 - (1 << lone)):
-                sel = code == c
-                if sel.any():
-                    tri = np.stack([edge(lone, k, sel) for k in others], 1)
-                    tris.append((tri, out_dir[sel]))
-        for a_, b_ in ((0, 1), (0, 2), (0, 3)):  # two and two: a quad
-            c_ = [k for k in range(4) if k not in (a_, b_)]
-            for c in ((1 << a_) | (1 << b_), 15 - ((1 << a_) | (1 << b_))):
-                sel = code == c
-                if sel.any():
-                    p0, p1 = edge(a_, c_[0], sel), edge(a_, c_[1], sel)
-                    p2, p3 = edge(b_, c_[1], sel), edge(b_, c_[0], sel)
-                    tris.append((np.stack([p0, p1, p2], 1), out_dir[sel]))
-                    tris.append((np.stack([p0, p2, p3], 1), out_dir[sel]))
-    verts, faces = [], []
-    nv = 0
-    for tri, od in tris:
-        nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
-        flip = (nrm * od).sum(1) < 0
-        tri = np.where(flip[:, None, None], tri[:, ::-1], tri)
-        n = tri.shape[0]
-        verts.append(tri.reshape(-1, 3))
-        faces.append(nv + np.arange(3 * n).reshape(n, 3))
-        nv += 3 * n
-    if not verts:
-        return np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64)
-    return np.concatenate(verts), np.concatenate(faces)
+def marching_tetrahedra_device(w, iso, bound_min, bound_max):
+    """Triangle soup of {w = iso} on the device (sn_mt_count / scan / sn_mt_emit):
+    float32 [T, 3, 3] world-space vertices, oriented from w < iso to w > iso."""
+    from . import _lib
+    from ._lib import call, ptr, stream_of, workspace
+    w = w.contiguous().to(torch.float32)
+    rx, ry, rz = (int(v) for v in w.shape)
+    dev = w.device
+    s = stream_of(dev)
+    n = (rx - 1) * (ry - 1) * (rz - 1)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    call("sn_mt_count", ptr(w), rx, ry, rz, float(iso), ptr(counts), s)
+    packed = torch.empty((n, 2), dtype=torch.int32, device=dev)
+    total = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = workspace(_lib.lib().sn_scan_workspace_bytes(n), dev)
+    call("sn_counts_to_packed_info", ptr(counts), n, ptr(packed), ptr(total), ptr(ws),
+         ws.numel(), s)
+    t = int(total.item())
+    tris = torch.empty((max(t, 1), 3, 3), dtype=torch.float32, device=dev)
+    bmin = (ctypes_floats(bound_min))
+    bmax = (ctypes_floats(bound_max))
+    if t:
+        call("sn_mt_emit", ptr(w), rx, ry, rz, float(iso), bmin, bmax, ptr(packed), ptr(tris), s)
+    return tris[:t]
 
 
-def extract_geometry(bound_min, bound_max, resolution, threshold, query_func):
-    """(vertices in world units, triangles) of {query_func = threshold}
-    (renderer.py:27-34; query_func is -sdf there)."""
-    u = extract_fields(bound_min, bound_max, resolution, query_func)
-    v, f = marching_tetrahedra(-u, -threshold)
-    bmin = np.asarray(bound_min, dtype=np.float64)
-    bmax = np.asarray(bound_max, dtype=np.float64)
-    v = v / (resolution - 1.0) * (bmax - bmin)[None, :] + bmin[None, :]
+def ctypes_floats(v):
+    import ctypes
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def extract_geometry(bound_min, bound_max, resolution, threshold, query_func, device=None):
+    """(vertices [3T, 3] in world units, triangles [T, 3]) of {query_func =
+    threshold} (renderer.py:27-34, where query_func is -sdf and the mesher is
+    PyMCubes' marching cubes; here marching tetrahedra on the GPU, the same
+    surface up to the triangulation inside each cube).  Triangles are oriented
+    towards query_func < threshold (outside for query_func = -sdf)."""
+    device = device or torch.device("cuda")
+    u = extract_fields_device(bound_min, bound_max, resolution, query_func, device)
+    tris = marching_tetrahedra_device(-u, -float(threshold), bound_min, bound_max)
+    v = tris.reshape(-1, 3).cpu().numpy().astype(np.float64)
+    f = np.arange(v.shape[0], dtype=np.int64).reshape(-1, 3)
     return v, f
