@@ -327,6 +327,15 @@ int sn_mt_emit(const float* values, int32_t rx, int32_t ry, int32_t rz, float is
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                  int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                  double eps, int32_t step, const int32_t* step_dev, void* stream);
+/* The same step, also refreshing an fp16 copy (copy16, nullable) of params   */
+/* [copy_offset, copy_offset + copy_n): the gather copy of the hash table     */
+/* (tcnn keeps its parameters' fp16 copy the same way).                       */
+int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                        int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
+                        double eps, int32_t step, const int32_t* step_dev, void* copy16,
+                        int64_t copy_offset, int64_t copy_n, void* stream);
+/* dst[i] = (fp16) src[i] (initialises such a copy). */
+int sn_to_half(const float* src, void* dst, int64_t n, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Device-count variants for an iteration with no host round trip.  Each    */
@@ -374,6 +383,12 @@ int sn_patch_points(const int64_t* ray_indices, const float* t_starts, const flo
                     const float* rays_d, const float* plane_n, int32_t patch_h,
                     int32_t patch_w, int32_t* diff_idx, int32_t* n_diff, float* points,
                     int32_t* n_eval, void* workspace, size_t workspace_bytes, void* stream);
+/* Hash-grid encode reading an fp16 copy of the table (blending in fp32).   */
+int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                          const void* params16, int32_t n_levels, int32_t n_features,
+                          int32_t log2_hashmap_size, int32_t base_resolution,
+                          float per_level_scale, int32_t n_active_levels, float* enc,
+                          void* stream);
 /* Hash-grid encode / backward / double backward over the first *n_dev      */
 /* points of an n_cap-point buffer (tcnn Encoding, as sn_hashgrid_*).       */
 int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,

@@ -71,6 +71,11 @@ SIGNATURES = {
     "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
     "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),
+    "sn_adam_step_copy16": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
+                                      C.c_double, _i32, _p, _p, _i64, _i64, _p]),
+    "sn_to_half": (C.c_int, [_p, _p, _i64, _p]),
+    "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
+                                        _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz,
                                          _p]),

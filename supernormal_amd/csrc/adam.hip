@@ -23,10 +23,13 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, con
   p = p - k.step_size * (m / denom);
 }
 
+// copy16 (nullable): fp16 copy of parameters [copy_off, copy_off + copy_n),
+// refreshed with the update (the hash table's gather copy).
 __global__ void __launch_bounds__(256)
 adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
             float* __restrict__ v, int64_t n, AdamK k, float lr_scalar,
-            const float* __restrict__ lr_dev, float bc1, const int32_t* __restrict__ step_dev) {
+            const float* __restrict__ lr_dev, float bc1, const int32_t* __restrict__ step_dev,
+            _Float16* __restrict__ copy16, int64_t copy_off, int64_t copy_n) {
   if (step_dev) {  // bias corrections of the step held on the device
     const double st = (double)*step_dev;
     bc1 = (float)(1.0 - pow(k.b1d, st));
@@ -49,9 +52,26 @@ adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restric
     p4[i] = pp;
     m4[i] = mm;
     v4[i] = vv;
+    if (copy16) {
+      const float q[4] = {pp.x, pp.y, pp.z, pp.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t c = 4 * i + e - copy_off;
+        if (c >= 0 && c < copy_n) copy16[c] = (_Float16)q[e];
+      }
+    }
   }
-  for (int64_t i = n4 * 4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+  for (int64_t i = n4 * 4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
     adam1(p[i], g[i], m[i], v[i], k);
+    if (copy16 && i - copy_off >= 0 && i - copy_off < copy_n) copy16[i - copy_off] = (_Float16)p[i];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+to_half_kernel(const float* __restrict__ src, _Float16* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (_Float16)src[i];
 }
 
 }  // namespace sn
@@ -60,10 +80,36 @@ using namespace sn;
 
 extern "C" {
 
+int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                        int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
+                        double eps, int32_t step, const int32_t* step_dev, void* copy16,
+                        int64_t copy_offset, int64_t copy_n, void* stream);
+
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                  int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                  double eps, int32_t step, const int32_t* step_dev, void* stream) {
+  return sn_adam_step_copy16(params, grads, exp_avg, exp_avg_sq, n, lr, lr_dev, beta1, beta2,
+                             eps, step, step_dev, nullptr, 0, 0, stream);
+}
+
+int sn_to_half(const float* src, void* dst, int64_t n, void* stream) {
   clear_error();
+  SN_REQUIRE(n >= 0, "sn_to_half: n < 0");
+  if (n == 0) return SN_OK;
+  SN_REQUIRE(src && dst, "sn_to_half: null pointer");
+  to_half_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
+      src, reinterpret_cast<_Float16*>(dst), n);
+  return check_launch("sn_to_half");
+}
+
+int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                        int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
+                        double eps, int32_t step, const int32_t* step_dev, void* copy16,
+                        int64_t copy_offset, int64_t copy_n, void* stream) {
+  clear_error();
+  SN_REQUIRE(!copy16 || (copy_offset >= 0 && copy_n >= 0 && copy_offset + copy_n <= n),
+             "sn_adam_step: copy range [%lld, %lld) outside [0, %lld)", (long long)copy_offset,
+             (long long)(copy_offset + copy_n), (long long)n);
   SN_REQUIRE(n >= 0 && (step >= 1 || step_dev), "sn_adam_step: bad n=%lld / step=%d",
              (long long)n, step);
   if (n == 0) return SN_OK;
@@ -85,7 +131,8 @@ int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_a
   k.inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
   k.step_size = 0.f;
   adam_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, as_stream(stream)>>>(
-      params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1, step_dev);
+      params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1, step_dev,
+      reinterpret_cast<_Float16*>(copy16), copy_offset, copy_n);
   return check_launch("sn_adam_step");
 }
 

@@ -46,10 +46,12 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
   return (int64_t)off;
 }
 
-template <int F>
+// Table element type TP: float (the parameters) or _Float16 (a copy refreshed
+// by the optimizer step: half the gather bytes; blending stays fp32).
+template <int F, typename TP = float>
 __global__ void __launch_bounds__(256)
 hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
-                const float* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
+                const TP* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= dev_count(n_cap, n_dev)) return;
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
@@ -62,11 +64,12 @@ hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
       uint32_t idx[8];
       float w[8], fr[3];
       hg_level(p, d, l, idx, w, fr);
-      const float* tab = params + (int64_t)d.offset[l] * F;
+      const TP* tab = params + (int64_t)d.offset[l] * F;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) acc[f] = fmaf(w[c], tab[(int64_t)idx[c] * F + f], acc[f]);
+        for (int f = 0; f < F; ++f)
+          acc[f] = fmaf(w[c], (float)tab[(int64_t)idx[c] * F + f], acc[f]);
       }
     }
 #pragma unroll
@@ -294,6 +297,26 @@ int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
                                                   as_stream(stream)>>>(x, n_cap, n_dev, params,
                                                                        d, enc));
   return check_launch("sn_hashgrid_fwd");
+}
+
+int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                          const void* params16, int32_t n_levels, int32_t n_features,
+                          int32_t log2_hashmap_size, int32_t base_resolution,
+                          float per_level_scale, int32_t n_active_levels, float* enc,
+                          void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_fwd16", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_fwd16: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && params16 && enc, "sn_hashgrid_fwd16: null pointer");
+  const _Float16* p16 = reinterpret_cast<const _Float16*>(params16);
+  SN_DISPATCH_F(n_features, (hash_fwd_kernel<kF, _Float16><<<(unsigned)ceil_div(n_cap, 256), 256,
+                                                              0, as_stream(stream)>>>(
+                                x, n_cap, n_dev, p16, d, enc)));
+  return check_launch("sn_hashgrid_fwd16");
 }
 
 int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_levels,

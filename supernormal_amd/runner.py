@@ -102,15 +102,10 @@ class Runner:
                                     self.renderer.occupancy_grid, self.gradient_method,
                                     self.max_samples_per_ray, self.normal_weight,
                                     self.eikonal_weight, self.mask_weight, self.loss_type)
+        self.rng_seed = rng_seed
         self.sf_step = None
         if fused and self.step_mode != "fused":
-            from .step import SyncFreeStep
-            self.sf_step = SyncFreeStep(
-                self.sdf_network, self.deviation_network, self.renderer.occupancy_grid,
-                self.optimizer, self.dataset, self.batch_size, self.patch_size,
-                self.max_samples_per_ray, self.normal_weight, self.eikonal_weight,
-                self.mask_weight, self.loss_type, precision=self.mlp_precision,
-                device=self.device, seed=rng_seed)
+            self._make_sync_free_step()
         if rng_seed is not None:
             # patch sampling and occupancy sampling from generators of this run only
             g = torch.Generator(device=self.device)
@@ -120,6 +115,15 @@ class Runner:
         self.history = []
         if is_continue:
             self.load_checkpoint()
+
+    def _make_sync_free_step(self):
+        from .step import SyncFreeStep
+        self.sf_step = SyncFreeStep(
+            self.sdf_network, self.deviation_network, self.renderer.occupancy_grid,
+            self.optimizer, self.dataset, self.batch_size, self.patch_size,
+            self.max_samples_per_ray, self.normal_weight, self.eikonal_weight,
+            self.mask_weight, self.loss_type, precision=self.mlp_precision,
+            device=self.device, seed=self.rng_seed)
 
     # ------------------------------------------------------------------ train
     def update_learning_rate(self):
@@ -252,6 +256,9 @@ class Runner:
         torch.set_rng_state(ck["rng_cpu"])
         if ck.get("rng_cuda") is not None:
             torch.cuda.set_rng_state(ck["rng_cuda"], self.device)
+        if self.sf_step is not None:
+            # new parameter buffers: new step buffers and graph
+            self._make_sync_free_step()
 
 
 def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=5000, lr=None,

@@ -39,7 +39,7 @@ class SyncFreeStep:
     def __init__(self, sdf_network, deviation_network, occupancy_grid, flat, dataset,
                  batch_size, patch_size=3, max_samples_per_ray=128, normal_weight=1.0,
                  eikonal_weight=1.0, mask_weight=1.0, loss_type="l2", early_stop_eps=1e-8,
-                 precision="fp16", device="cuda", seed=None):
+                 precision="fp16", device="cuda", seed=None, half_table=None):
         if max_samples_per_ray <= 0:
             raise ValueError("SyncFreeStep needs max_samples_per_ray > 0 (it bounds the buffers)")
         if precision not in ("fp32", "fp16"):
@@ -56,6 +56,9 @@ class SyncFreeStep:
         self.l1 = loss_type == "l1"
         self.eps = float(early_stop_eps)
         self.precision = precision
+        # forward gathers from an fp16 copy of the hash table, refreshed by each
+        # Adam step (fp32 master table and gradients); default with precision fp16
+        self.half_table = (precision == "fp16") if half_table is None else bool(half_table)
         cfg = sdf_network.sdf_config()
         self.cfg = cfg
         self.E = cfg.E
@@ -146,6 +149,22 @@ class SyncFreeStep:
         self.ws_mlp = nb(wsb)
         n_mlp = lib.sn_mlp_n_params(3 + E, cfg.d_hidden, cfg.n_layers)
         self.g_mlp = torch.zeros(n_mlp, **f32)
+        self.table16 = None
+        if self.half_table:
+            table = self.net.encoding.params
+            self.table16 = torch.empty(table.numel(), dtype=torch.float16, device=dev)
+            k = [i for i, q in enumerate(self.flat.params) if q is table]
+            if not k:
+                raise ValueError("the hash table must be one of the optimised parameters")
+            self.table_off = self.flat.offsets[k[0]]
+            self.refresh_copies()
+
+    def refresh_copies(self):
+        """Re-derive the fp16 table copy from the parameters (after they were
+        changed outside the step, e.g. a checkpoint load)."""
+        if self.table16 is not None:
+            t = self.net.encoding.params
+            call("sn_to_half", ptr(t), ptr(self.table16), t.numel(), stream_of(self.dev))
 
     # ------------------------------------------------------------- one iteration
     def _sample(self):
@@ -181,8 +200,12 @@ class SyncFreeStep:
 
     def _sdf_fwd(self, pts, enc, sdf, cap, n_dev, mlp, s):
         cfg = self.cfg
-        call("sn_hashgrid_fwd_dev", ptr(pts), cap, ptr(n_dev), ptr(self.net.encoding.params),
-             *self._hash_args(), ptr(enc), s)
+        if self.table16 is not None:
+            call("sn_hashgrid_fwd16_dev", ptr(pts), cap, ptr(n_dev), ptr(self.table16),
+                 *self._hash_args(), ptr(enc), s)
+        else:
+            call("sn_hashgrid_fwd_dev", ptr(pts), cap, ptr(n_dev),
+                 ptr(self.net.encoding.params), *self._hash_args(), ptr(enc), s)
         if self.precision == "fp16":
             call("sn_mlp16_fwd", ptr(pts), ptr(enc), cfg.E, cap, ptr(n_dev), ptr(mlp),
                  cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), s)
@@ -260,8 +283,13 @@ class SyncFreeStep:
         # Adam with lr and step count on the device
         self.adam_t += 1
         f = self.flat
-        call("sn_adam_step", ptr(f.data), ptr(f.grad), ptr(f.exp_avg), ptr(f.exp_avg_sq), f.n,
-             0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0, ptr(self.adam_t), s)
+        if self.table16 is not None:
+            call("sn_adam_step_copy16", ptr(f.data), ptr(f.grad), ptr(f.exp_avg),
+                 ptr(f.exp_avg_sq), f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0,
+                 ptr(self.adam_t), ptr(self.table16), self.table_off, self.table16.numel(), s)
+        else:
+            call("sn_adam_step", ptr(f.data), ptr(f.grad), ptr(f.exp_avg), ptr(f.exp_avg_sq),
+                 f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0, ptr(self.adam_t), s)
 
     # ------------------------------------------------------------------- driver
     def step(self, lr, step_size, use_graph=True, inputs=None):

@@ -137,3 +137,38 @@ def test_full_size_properties(dev):
         e.params.copy_(p1 + p2)
         o3 = e(x)
         torch.testing.assert_close(o3, o1 + o2, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("name,n_active", [("cfg2", 16), ("cfg1", 5)])
+def test_forward_from_fp16_copy(name, n_active, dev):
+    """The fp16-copy forward equals the fp32 forward of the rounded table
+    (same gathers, same fp32 blending), bit for bit; and the copy kept by the
+    optimizer step (sn_adam_step_copy16) is the rounding of the updated table."""
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS[name]
+    e = _enc(cfg, dev, seed=4)
+    n = 50_000
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = (torch.rand(n, 3, generator=g) * 2 - 1).to(dev)
+    args = (cfg["n_levels"], cfg["n_features_per_level"], cfg["log2_hashmap_size"],
+            cfg["base_resolution"], cfg["per_level_scale"], n_active)
+    t16 = torch.empty(e.params.numel(), dtype=torch.float16, device=dev)
+    st = stream_of(dev)
+    call("sn_to_half", ptr(e.params), ptr(t16), e.params.numel(), st)
+    assert torch.equal(t16, e.params.detach().half())
+    rounded = t16.float()
+    L, F = cfg["n_levels"], cfg["n_features_per_level"]
+    a = torch.empty(n, L * F, device=dev)
+    b = torch.empty(n, L * F, device=dev)
+    n_dev = torch.tensor([n], dtype=torch.int32, device=dev)
+    call("sn_hashgrid_fwd16_dev", ptr(x), n, ptr(n_dev), ptr(t16), *args, ptr(a), st)
+    call("sn_hashgrid_fwd", ptr(x), n, ptr(rounded), *args, ptr(b), st)
+    assert torch.equal(a, b)
+    # Adam refreshes the copy of a sub-range of a larger flat buffer
+    m = e.params.numel()
+    flat = torch.randn(m + 1003, generator=g).to(dev)
+    grad = torch.randn(m + 1003, generator=g).to(dev)
+    mom1, mom2 = torch.zeros_like(flat), torch.zeros_like(flat)
+    call("sn_adam_step_copy16", ptr(flat), ptr(grad), ptr(mom1), ptr(mom2), flat.numel(), 1e-2,
+         None, 0.9, 0.999, 1e-8, 1, None, ptr(t16), 1001, m, st)
+    assert torch.equal(t16, flat[1001:1001 + m].half())

@@ -34,7 +34,13 @@ def _state(flat):
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_sync_free_matches_fused_step(dev, precision):
     r = _runner(dev, precision)
-    sf, fs, flat = r.sf_step, r.fused_step, r.optimizer
+    fs, flat = r.fused_step, r.optimizer
+    # FusedStep gathers from the fp32 table: compare without the fp16 table copy
+    from supernormal_amd.step import SyncFreeStep
+    sf = SyncFreeStep(r.sdf_network, r.deviation_network, r.renderer.occupancy_grid, flat,
+                      r.dataset, r.batch_size, r.patch_size, r.max_samples_per_ray,
+                      r.normal_weight, r.eikonal_weight, r.mask_weight, r.loss_type,
+                      precision=precision, device=dev, half_table=False)
     g = torch.Generator(device=dev).manual_seed(7)
     o, d, pn, vi, nt, m = r.dataset.gen_random_patches(r.batch_size, generator=g)
     near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
