@@ -184,7 +184,9 @@ class Server:
         self.make_job = make_job or (lambda spec: Job(spec, self.device, precision))
         self.clock, self.sleep = clock, sleep
 
-    def run(self, specs: List[JobSpec], log=None):
+    def run(self, specs: List[JobSpec], log=None, on_done=None):
+        """Serve `specs`; `on_done(job)` sees each finished job before its state
+        is released."""
         queue = sorted(specs, key=lambda s: s.arrival_s)
         running, records = [], []
         used = 0
@@ -228,6 +230,8 @@ class Server:
                 records.append(rec)
                 if log:
                     log(rec)
+                if on_done:
+                    on_done(job)
                 job.release()
         t_end = self.clock() - t0
         ttm = np.array([r["time_to_mesh_s"] for r in records])

@@ -1,0 +1,56 @@
+"""Serving on the GPU: three small jobs run alone and together in the request
+loop give the same results within tolerance.
+
+Each job samples from generators of its own, so running beside other jobs
+changes nothing but the order of fp32 atomics in the hash-grid backward; Adam
+turns that order noise into small parameter differences for entries whose
+gradient nearly cancels, so the comparison is on the field, not the table:
+SDF at 4096 points |d| <= 2e-3 and the per-evaluation Chamfer within 10%."""
+import pytest
+import torch
+
+from supernormal_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    return [serving.JobSpec(job_id=i, seed=100 + i, n_views=8, resolution=128,
+                            log2_hashmap_size=15 + i, arrival_s=0.0, batch_size=256,
+                            max_iters=60, chamfer_tol=0.0, eval_every=30, mesh_resolution=48)
+            for i in range(3)]
+
+
+def _run(dev, specs, max_running):
+    g = torch.Generator(device="cpu").manual_seed(0)
+    pts = (torch.rand(4096, 3, generator=g) * 1.6 - 0.8).to(dev)
+    fields = {}
+
+    def keep(job):
+        with torch.no_grad():
+            fields[job.spec.job_id] = (job.runner.sdf_network.fused_sdf(pts).cpu(),
+                                       [h[1] for h in job.history])
+
+    srv = serving.Server(device=dev, hbm_budget_bytes=64 << 30, max_running=max_running,
+                         chunk=5)
+    out = srv.run(specs, on_done=keep)
+    return fields, out
+
+
+def test_jobs_same_alone_and_in_the_mix(dev):
+    alone = {}
+    for s in _specs():
+        f, _ = _run(dev, [s], 1)
+        alone.update(f)
+    mixed, out = _run(dev, _specs(), 3)
+    assert out["peak_running"] == 3
+    assert len(out["records"]) == 3
+    for j in alone:
+        sdf_a, cd_a = alone[j]
+        sdf_m, cd_m = mixed[j]
+        assert float((sdf_a - sdf_m).abs().max()) <= 2e-3
+        assert len(cd_a) == len(cd_m)
+        for a, b in zip(cd_a, cd_m):
+            assert abs(a - b) <= 0.1 * a + 1e-4
+    for r in out["records"]:
+        assert r["time_to_mesh_s"] > 0 and r["iters"] == 60
