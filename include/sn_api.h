@@ -349,10 +349,14 @@ int sn_pack_info_dev(const int64_t* ray_indices, int64_t n_cap, const int32_t* n
                      int32_t n_rays, int32_t* packed_info, int32_t* n_bad, void* workspace,
                      size_t workspace_bytes, void* stream);
 /* Compaction by a keep mask (ray_marching.py:216-220) over *n_dev samples. */
+/* With packed_in (the rays' packed_info before compaction, nullable) it    */
+/* also writes packed_out = pack_info of the kept samples (pack.py:46-77)   */
+/* without counting them again.                                             */
 int sn_compact_samples_dev(const uint8_t* keep, int32_t n_cap, const int32_t* n_dev,
                            const int64_t* ray_indices, const float* t_starts,
                            const float* t_ends, int64_t* out_ray_indices, float* out_t_starts,
-                           float* out_t_ends, int32_t* n_out, void* workspace,
+                           float* out_t_ends, int32_t* n_out, const int32_t* packed_in,
+                           int32_t n_rays, int32_t* packed_out, void* workspace,
                            size_t workspace_bytes, void* stream);
 
 /* Visibility pre-pass points (renderer.py:80-100): for the marched centre   */

@@ -77,8 +77,8 @@ SIGNATURES = {
     "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
                                         _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
-    "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz,
-                                         _p]),
+    "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
+                                         _p, _p, _sz, _p]),
     "sn_step_workspace_bytes": (_sz, [_i64]),
     "sn_centre_points": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i32, _p, _p, _p, _p, _p,
                                    _sz, _p]),

@@ -19,7 +19,8 @@
 // for fp16, ~16 bits for bf16) and both parts are multiplied, so no per-point
 // rounding noise enters the SDF at fp16 resolution.  The forward pass runs on
 // fp16; the backward on bf16, whose range holds the small loss gradients
-// without scaling.
+// without scaling, with single-part operands (gradients need ~2^-9 per term,
+// not the forward's smoothness), recomputing the forward from hi parts only.
 //
 // Weight gradients need the points on the K axis, i.e. the transpose of the
 // register layout: each group's gradient and activation blocks are written to
@@ -236,7 +237,9 @@ struct Fwd {
   float h2[MB][4], d2[MB][4];  // softplus(z1), softplus'(z1) (NH == 2)
 };
 
-template <int H, int NH, int KB0, bool kDeriv>
+// kLo: also multiply the lo parts of the activations (the forward whose values
+// are used); the backward's recompute needs only the hi parts.
+template <int H, int NH, int KB0, bool kDeriv, bool kLo = true>
 __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1, const float* vec,
                                               const float (&h0)[KB0][8], float beta, float inv_beta,
                                               Fwd<H, NH, KB0>& F) {
@@ -258,7 +261,7 @@ __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1
 #pragma unroll
     for (int kb = 0; kb < KB0; ++kb) {
       const half8 w = fW0[(mb * KB0 + kb) * 64 + l];
-      z = mma_h(w, bl[kb], z);
+      if (kLo) z = mma_h(w, bl[kb], z);
       z = mma_h(w, bh[kb], z);
     }
 #pragma unroll
@@ -283,7 +286,7 @@ __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1
 #pragma unroll
       for (int kb = 0; kb < C::KBH; ++kb) {
         const half8 w = fW1[(mb * C::KBH + kb) * 64 + l];
-        z = mma_h(w, cl[kb], z);
+        if (kLo) z = mma_h(w, cl[kb], z);
         z = mma_h(w, ch[kb], z);
       }
 #pragma unroll
@@ -378,7 +381,8 @@ __device__ __forceinline__ void tile_operand(const uint32_t* t, int row, int k, 
     lo[j] = __builtin_bit_cast(__bf16, (uint16_t)(d[j] >> 16));
   }
 }
-// acc += A[16a .. 16a+15][pts] * B[16b .. 16b+15][pts]^T over the step's points.
+// acc += A[16a .. 16a+15][pts] * B[16b .. 16b+15][pts]^T over the step's points
+// (bf16 products of the hi parts: a gradient sum, ~2^-9 relative per term).
 __device__ __forceinline__ floatx4 tile_mma(const uint32_t* tA, int a, const uint32_t* tB, int b,
                                             floatx4 acc) {
   const int r = threadIdx.x & 15;
@@ -387,8 +391,6 @@ __device__ __forceinline__ floatx4 tile_mma(const uint32_t* tA, int a, const uin
     bf16x4 ah, al, bh, bl;
     tile_operand(tA, 16 * a + r, k, ah, al);
     tile_operand(tB, 16 * b + r, k, bh, bl);
-    acc = mma_b4(al, bh, acc);
-    acc = mma_b4(ah, bl, acc);
     acc = mma_b4(ah, bh, acc);
   }
   return acc;
@@ -452,7 +454,7 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
     for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, E, vec4, p, ok, kb, q, h0[kb]);
     const float gs = ok ? g_sdf[p] : 0.f;
     Fwd<H, NH, KB0> F;
-    forward_group<H, NH, KB0, true>(fW0, fW1, vec, h0, beta, inv_beta, F);
+    forward_group<H, NH, KB0, true, false>(fW0, fW1, vec, h0, beta, inv_beta, F);
     if (q == 0) dbo += gs;
     float gz[MB][4];
 #pragma unroll
@@ -488,7 +490,6 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
 #pragma unroll
         for (int kb = 0; kb < KBH; ++kb) {
           const bf16x8 w = fW1T[(mi * KBH + kb) * 64 + l];
-          acc = mma_b(w, gl[kb], acc);
           acc = mma_b(w, gh[kb], acc);
         }
 #pragma unroll
@@ -519,7 +520,6 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
 #pragma unroll
         for (int kb = 0; kb < KBH; ++kb) {
           const bf16x8 w = fW0T[(md * KBH + kb) * 64 + l];
-          acc = mma_b(w, gl[kb], acc);
           acc = mma_b(w, gh[kb], acc);
         }
         if (ok) {

@@ -25,11 +25,11 @@ __device__ __forceinline__ float calc_dt(float t, float cone, float dt_min, floa
   return fmaxf(dt_min, fminf(t * cone, dt_max));
 }
 
-__device__ __forceinline__ bool occupied_at(const float xyz[3], const RoiGrid& g,
-                                            const uint8_t* __restrict__ grid) {
+// Cell index of a point, or -1 outside the roi (then unoccupied).
+__device__ __forceinline__ int grid_index(const float xyz[3], const RoiGrid& g) {
   if (xyz[0] < g.mn[0] || xyz[0] > g.mx[0] || xyz[1] < g.mn[1] || xyz[1] > g.mx[1] ||
       xyz[2] < g.mn[2] || xyz[2] > g.mx[2])
-    return false;
+    return -1;
   int ix[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -38,9 +38,25 @@ __device__ __forceinline__ bool occupied_at(const float xyz[3], const RoiGrid& g
     v = max(0, min(v, g.res[d] - 1));
     ix[d] = v;
   }
-  int idx = ix[0] * (g.res[1] * g.res[2]) + ix[1] * g.res[2] + ix[2];
-  return grid[idx] != 0;
+  return ix[0] * (g.res[1] * g.res[2]) + ix[1] * g.res[2] + ix[2];
 }
+
+// Occupancy lookups along one ray: consecutive samples usually fall in the
+// same cell (the step is far below the cell size), so the last cell's value
+// is kept and memory is read only when the cell changes.
+struct CellCache {
+  int idx = -2;
+  bool occ = false;
+  __device__ __forceinline__ bool lookup(const float xyz[3], const RoiGrid& g,
+                                         const uint8_t* __restrict__ grid) {
+    const int i = grid_index(xyz, g);
+    if (i != idx) {
+      idx = i;
+      occ = i >= 0 && grid[i] != 0;
+    }
+    return occ;
+  }
+};
 
 __device__ __forceinline__ float distance_to_next_voxel(const float xyz[3], const float dir[3],
                                                         const float inv_dir[3],
@@ -130,11 +146,12 @@ sn::ray_march_kernel_roi(const float* __restrict__ rays_o, const float* __restri
   float t1 = t0 + dt;
   float t_mid = (t0 + t1) * 0.5f;
   int64_t iter = 0;
+  CellCache cells;
   while (t_mid < far && j < cap && iter < kMaxIter) {
     float xyz[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) xyz[d] = o[d] + t_mid * dir[d];
-    if (occupied_at(xyz, g, grid)) {
+    if (cells.lookup(xyz, g, grid)) {
       if (kMode != kCount) {
         t_starts[base + j] = t0;
         t_ends[base + j] = t1;

@@ -236,6 +236,24 @@ __global__ void compact_kernel(const uint8_t* keep, int64_t n_cap, const int32_t
   }
 }
 
+// packed_out[r] = (first kept sample of ray r, kept count) from the rays'
+// pre-compaction packed_in and the exclusive keep offsets.
+__global__ void compact_packed_kernel(const int32_t* __restrict__ packed_in, int32_t n_rays,
+                                      int64_t n_cap, const int32_t* __restrict__ n_dev,
+                                      const int32_t* __restrict__ offs,
+                                      const int32_t* __restrict__ total,
+                                      int32_t* __restrict__ packed_out) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rays) return;
+  const int64_t n = eff_n(n_cap, n_dev);
+  const int64_t b = packed_in[2 * r], c = packed_in[2 * r + 1];
+  const int32_t t = *total;
+  const int32_t s = b < n ? offs[b] : t;
+  const int32_t e = b + c < n ? offs[b + c] : t;
+  packed_out[2 * r] = s;
+  packed_out[2 * r + 1] = e - s;
+}
+
 __global__ void nonzero_write_kernel(const uint8_t* keep, int64_t n, const int32_t* offs,
                                      int64_t* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -306,7 +324,8 @@ size_t sn_compact_workspace_bytes(int64_t n) {
 int sn_compact_samples_dev(const uint8_t* keep, int32_t n_cap, const int32_t* n_dev,
                            const int64_t* ray_indices, const float* t_starts,
                            const float* t_ends, int64_t* out_ray_indices, float* out_t_starts,
-                           float* out_t_ends, int32_t* n_out, void* ws, size_t ws_bytes,
+                           float* out_t_ends, int32_t* n_out, const int32_t* packed_in,
+                           int32_t n_rays, int32_t* packed_out, void* ws, size_t ws_bytes,
                            void* stream) {
   clear_error();
   SN_REQUIRE(n_cap >= 0, "sn_compact_samples: n < 0");
@@ -325,6 +344,9 @@ int sn_compact_samples_dev(const uint8_t* keep, int32_t n_cap, const int32_t* n_
   compact_kernel<<<grid_for(n_cap, 256), 256, 0, s>>>(keep, n_cap, n_dev, offs, ray_indices,
                                                       t_starts, t_ends, out_ray_indices,
                                                       out_t_starts, out_t_ends);
+  if (packed_in && packed_out && n_rays > 0)
+    compact_packed_kernel<<<(unsigned)ceil_div(n_rays, 256), 256, 0, s>>>(
+        packed_in, n_rays, n_cap, n_dev, offs, n_out, packed_out);
   return check_launch("sn_compact_samples");
 }
 
@@ -333,7 +355,8 @@ int sn_compact_samples(const uint8_t* keep, int32_t n, const int64_t* ray_indice
                        float* out_t_starts, float* out_t_ends, int32_t* n_out, void* ws,
                        size_t ws_bytes, void* stream) {
   return sn_compact_samples_dev(keep, n, nullptr, ray_indices, t_starts, t_ends, out_ray_indices,
-                                out_t_starts, out_t_ends, n_out, ws, ws_bytes, stream);
+                                out_t_starts, out_t_ends, n_out, nullptr, 0, nullptr, ws,
+                                ws_bytes, stream);
 }
 
 int sn_occ_nonzero(const uint8_t* binary, int32_t n_cells, int64_t* out_idx, int32_t* count,

@@ -11,8 +11,9 @@ sequence of launches on one stream:
   patch sampling (dataset_loader.py:223-277, device RNG)
   -> stratified jitter, sn_ray_march_padded / scan / sn_march_gather (one
      march per ray instead of count + fill, renderer.py:66-79)
-  -> sn_centre_points, SDF forward, sn_alpha_visibility, sn_compact_samples_dev,
-     sn_pack_info_dev (the visibility pre-pass, renderer.py:80-135)
+  -> sn_centre_points, SDF forward, sn_alpha_visibility, sn_compact_samples_dev
+     (with the kept samples' packed_info; the visibility pre-pass,
+     renderer.py:80-135)
   -> sn_patch_points (patch expansion, renderer.py:142-159)
   -> hash-grid + MLP forward on every patch sample (renderer.py:161)
   -> sn_render_step (weights, normals, losses and their backward)
@@ -140,7 +141,6 @@ class SyncFreeStep:
         self.ws_scan = nb(lib.sn_scan_workspace_bytes(B))
         self.ws_step = nb(lib.sn_step_workspace_bytes(c0))
         self.ws_compact = nb(lib.sn_compact_workspace_bytes(c0))
-        self.ws_pack = nb(lib.sn_pack_info_workspace_bytes(B))
         cfg = self.cfg
         if self.precision == "fp16":
             wsb = lib.sn_mlp16_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers)
@@ -247,11 +247,11 @@ class SyncFreeStep:
         self._sdf_fwd(self.pts0, self.enc0, self.sdf0, 2 * c0, self.ne0, mlp_d, s)
         call("sn_alpha_visibility", ptr(self.packed0), B, ptr(self.sdf0), ptr(self.didx0), c0,
              ptr(self.n0), ptr(self.inv_s), self.eps, ptr(self.keep), s)
+        # compaction also derives the packed_info of the kept samples
         call("sn_compact_samples_dev", ptr(self.keep), c0, ptr(self.n0), ptr(self.ri0),
              ptr(self.t00), ptr(self.t10), ptr(self.ri), ptr(self.t0), ptr(self.t1),
-             ptr(self.ns), ptr(self.ws_compact), self.ws_compact.numel(), s)
-        call("sn_pack_info_dev", ptr(self.ri), c0, ptr(self.ns), B, ptr(self.packed), None,
-             ptr(self.ws_pack), self.ws_pack.numel(), s)
+             ptr(self.ns), ptr(self.packed0), B, ptr(self.packed), ptr(self.ws_compact),
+             self.ws_compact.numel(), s)
         # patch expansion and the SDF on every patch sample
         call("sn_patch_points", ptr(self.ri), ptr(self.t0), ptr(self.t1), c0, ptr(self.ns),
              ptr(self.rays_o), ptr(self.rays_d), ptr(self.plane_n), self.Ph, self.Pw,
@@ -300,6 +300,8 @@ class SyncFreeStep:
         on the inputs of the previous iteration (eager only)."""
         self.lr.fill_(float(lr))
         self.step_size.fill_(float(step_size))
+        # the optimizer's step count is the host's (other drivers may have stepped it)
+        self.adam_t.fill_(int(self.flat.step_count))
         if not use_graph or inputs is not None:
             self._iteration(inputs)
             self.flat.step_count += 1

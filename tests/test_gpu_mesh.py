@@ -6,10 +6,35 @@ sphere and the mesh is closed (every edge shared by two triangles)."""
 import numpy as np
 import pytest
 import torch
+from scipy.spatial import cKDTree
 
 from oracle import mesh_oracle
 
 pytestmark = pytest.mark.gpu
+
+
+def _same_triangles(got, ref, tol=2e-5):
+    """Every triangle of `got` has a triangle of `ref` with the same vertices in
+    the same cyclic order (orientation) within tol, and the counts agree."""
+    assert got.shape == ref.shape
+    rots = np.concatenate([np.roll(ref, k, axis=1) for k in range(3)]).reshape(-1, 9)
+    d, _ = cKDTree(rots).query(got.reshape(-1, 9))
+    assert d.max() < tol, d.max()
+
+
+def _weld(v, tol=1e-5):
+    """Vertex ids after merging points closer than tol."""
+    ids = np.arange(len(v))
+
+    def root(a):
+        while ids[a] != a:
+            a = ids[a]
+        return a
+
+    for i, j in cKDTree(v).query_pairs(tol):
+        a, b = root(i), root(j)
+        ids[max(a, b)] = min(a, b)
+    return np.array([root(i) for i in range(len(v))])
 
 
 def _grid(r, f):
@@ -25,18 +50,18 @@ def test_sphere_matches_oracle(dev, r):
     got = marching_tetrahedra_device(torch.as_tensor(u, device=dev), 0.0, [-1, -1, -1],
                                      [1, 1, 1]).cpu().numpy()
     ref = mesh_oracle.triangles(u, 0.0, [-1, -1, -1], [1, 1, 1])
-    assert got.shape == ref.shape
-    a, b = mesh_oracle.canonical(got), mesh_oracle.canonical(ref)
-    np.testing.assert_allclose(a, b, atol=2e-5)
+    _same_triangles(got, ref)
     rad = np.linalg.norm(got.reshape(-1, 3), axis=1)
     assert np.abs(rad - 0.6).max() < 2.0 / (r - 1)
-    # closed and consistently oriented: every directed edge appears once, its reverse once
-    keys = np.round(got, 5)
+    # closed and consistently oriented: after welding, every directed edge's
+    # reverse appears exactly as often as the edge
+    ids = _weld(got.reshape(-1, 3)).reshape(-1, 3)
     edges = {}
-    for tri in keys:
+    for t in ids:
         for i in range(3):
-            e = (tuple(tri[i]), tuple(tri[(i + 1) % 3]))
-            edges[e] = edges.get(e, 0) + 1
+            e = (t[i], t[(i + 1) % 3])
+            if e[0] != e[1]:
+                edges[e] = edges.get(e, 0) + 1
     assert all(edges.get((b_, a_), 0) == c for (a_, b_), c in edges.items())
     # outward: positive signed volume ~ 4/3 pi 0.6^3
     vol = np.einsum("ij,ij->i", got[:, 0], np.cross(got[:, 1], got[:, 2])).sum() / 6.0
@@ -50,8 +75,8 @@ def test_random_field_all_cases(dev):
     got = marching_tetrahedra_device(torch.as_tensor(u, device=dev), 0.1, [-1, -2, 0],
                                      [1, 1, 3]).cpu().numpy()
     ref = mesh_oracle.triangles(u, 0.1, [-1, -2, 0], [1, 1, 3])
-    assert got.shape == ref.shape and got.shape[0] > 100
-    np.testing.assert_allclose(mesh_oracle.canonical(got), mesh_oracle.canonical(ref), atol=2e-5)
+    assert got.shape[0] > 100
+    _same_triangles(got, ref)
 
 
 def test_extract_geometry_of_sdf(dev):

@@ -206,11 +206,17 @@ def test_compact_samples_dev_prefix(dev, n):
     n_out = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = torch.empty(lib().sn_compact_workspace_bytes(cap), dtype=torch.uint8, device=dev)
     kt, rt, t0t, t1t = T(keep), T(ri), T(t0), T(t1)
+    # packed_info of the rays before compaction (sorted ray indices, 999 rays)
+    n_rays = 999
+    pin = torch.as_tensor(c_oracle.pack_info(ri[:n], n_rays), device=dev)
+    pout = torch.full((n_rays, 2), -7, dtype=torch.int32, device=dev)
     call("sn_compact_samples_dev", ptr(kt), cap, ptr(n_dev), ptr(rt), ptr(t0t), ptr(t1t),
-         ptr(ori), ptr(o0), ptr(o1), ptr(n_out), ptr(ws), ws.numel(), stream_of(dev))
+         ptr(ori), ptr(o0), ptr(o1), ptr(n_out), ptr(pin), n_rays, ptr(pout), ptr(ws),
+         ws.numel(), stream_of(dev))
     k = keep[:n].astype(bool)
     m = int(k.sum())
     assert int(n_out) == m
+    np.testing.assert_array_equal(pout.cpu().numpy(), c_oracle.pack_info(ri[:n][k], n_rays))
     np.testing.assert_array_equal(ori[:m].cpu().numpy(), ri[:n][k])
     np.testing.assert_array_equal(o0[:m].cpu().numpy(), t0[:n][k])
     np.testing.assert_array_equal(o1[:m].cpu().numpy(), t1[:n][k])

@@ -86,6 +86,7 @@ def test_graph_replays_match_eager(dev):
         # eager work between replays, then the same step eagerly on the replay's inputs
         for t, v in zip((flat.data, flat.exp_avg, flat.exp_avg_sq), before):
             t.copy_(v)
+        sf.refresh_copies()  # the fp16 table copy follows the restored parameters
         sf.adam_t.copy_(t_before)
         flat.step_count -= 1
         losses_e = sf.step(lr, step, use_graph=False, inputs="keep").clone()
