@@ -4,8 +4,11 @@ Workload (the full-size single-object configuration): seeded smooth union of
 a torus (R 0.4, r 0.12), a 0.3-side box and 3 spheres; 20 ring + elevation
 cameras at 612x512 with N(0, 0.02) normal noise + masks; hash grid 16 levels
 x 2 features, T = 2^19, base 16, x1.38; SDF MLP 64 wide, 2 hidden layers;
-2048 patches of 3x3 (18 432 rays) per iteration; occupancy grid 128^3; <= 128
-samples per ray; 5000 iterations with the reference's schedules.  A "step" is
+2048 patches of 3x3 (18 432 rays) per iteration; occupancy grid 128^3;
+marching with a capacity of 1024 samples per centre ray (the reference is
+unbounded; a cap of 128 truncates rays before the surface late in training and
+every path diverges, --max-samples 128 measures it); 5000 iterations with the
+reference's schedules.  A "step" is
 one training iteration (occupancy update when due, patch sampling, marching,
 rendering, losses, backward, Adam) on the fused HIP path: by default with no
 host round trip and replayed as one HIP graph (--mode graph), the MLP on the
@@ -108,6 +111,8 @@ def main(argv=None):
     ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
     ap.add_argument("--roofline-iters", type=int, default=20)
+    ap.add_argument("--max-samples", type=int, default=None,
+                    help="samples per centre ray capacity (default: the workload's)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs on a "
@@ -120,7 +125,7 @@ def main(argv=None):
     mode = a.mode if a.method == "dfd" else "fused"
     conf = synthetic_conf(a.workload, gradient_method=a.method,
                           end_iter=max(a.steps + a.warmup, 1), mlp_precision=a.precision,
-                          step_mode=mode)
+                          step_mode=mode, max_samples_per_ray=a.max_samples)
     runner = Runner(conf, fused=True, device=dev)
     sf = runner.sf_step
     grid = runner.sdf_network.encoding.grid
@@ -191,7 +196,8 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{a.workload}: torus+box+3 spheres, 20 views 612x512, hash 16x2 "
                                f"T=2^19 base 16 x1.38, MLP 64x2, 2048 3x3 patches (18432 rays), "
-                               f"occupancy 128^3, <=128 samples/ray, gradient {a.method}, "
+                               f"occupancy 128^3, {runner.max_samples_per_ray} samples/ray "
+                               f"capacity, gradient {a.method}, "
                                f"mode {mode}, mlp {a.precision}",
                    "global_batch": runner.batch_size * 9, "iterations": a.warmup + a.steps,
                    "parallelism": "single GPU"},

@@ -48,12 +48,10 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
 
 // Table element type TP: float (the parameters) or _Float16 (a copy refreshed
 // by the optimizer step: half the gather bytes; blending stays fp32).
-template <int F, typename TP = float>
-__global__ void __launch_bounds__(256)
-hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
-                const TP* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= dev_count(n_cap, n_dev)) return;
+template <int F, typename TP>
+__device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restrict__ x,
+                                               const TP* __restrict__ params,
+                                               const HashGridDesc& d, float* __restrict__ enc) {
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   float* out = enc + i * (int64_t)(d.n_levels * F);
   for (int l = 0; l < d.n_levels; ++l) {
@@ -75,6 +73,20 @@ hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
 #pragma unroll
     for (int f = 0; f < F; ++f) out[l * F + f] = acc[f];
   }
+}
+
+// Launches are sized by a capacity and bounded (kHashBlocks); the lanes
+// stride over the count read from the device.
+constexpr int kHashBlocks = 256 * 16;
+
+template <int F, typename TP = float>
+__global__ void __launch_bounds__(256)
+hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                const TP* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
+  const int64_t n = dev_count(n_cap, n_dev);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    hash_fwd_point<F, TP>(i, x, params, d, enc);
 }
 
 // Wavefront-aggregated atomic add of F values at table row idx: lanes that
@@ -107,15 +119,13 @@ __device__ __forceinline__ void wave_row_atomic_add(float* __restrict__ tab, uin
 constexpr float kAggScale = 192.f;
 
 template <int F>
-__global__ void __launch_bounds__(256)
-hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
-                const float* __restrict__ params, HashGridDesc d,
-                const float* __restrict__ genc, float* __restrict__ gparams,
-                float* __restrict__ gx, int aggregate) {
-  const int64_t n = dev_count(n_cap, n_dev);
-  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  // whole wavefronts past the end leave together (uniform branch)
-  if ((i0 & ~(int64_t)63) >= n) return;
+__device__ __forceinline__ void hash_bwd_point(int64_t i0, int64_t n,
+                                               const float* __restrict__ x,
+                                               const float* __restrict__ params,
+                                               const HashGridDesc& d,
+                                               const float* __restrict__ genc,
+                                               float* __restrict__ gparams,
+                                               float* __restrict__ gx, int aggregate) {
   const bool act = i0 < n;
   const int64_t i = act ? i0 : 0;  // inactive lanes stay for the wavefront collectives
   if (!aggregate && !act) return;
@@ -169,6 +179,20 @@ hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
   }
 }
 
+template <int F>
+__global__ void __launch_bounds__(256)
+hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                const float* __restrict__ params, HashGridDesc d,
+                const float* __restrict__ genc, float* __restrict__ gparams,
+                float* __restrict__ gx, int aggregate) {
+  const int64_t n = dev_count(n_cap, n_dev);
+  // the stride is a multiple of 64, so a wavefront's lanes stay together and
+  // leave the loop together (the aggregation uses wavefront collectives)
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; (i0 & ~(int64_t)63) < n;
+       i0 += (int64_t)gridDim.x * blockDim.x)
+    hash_bwd_point<F>(i0, n, x, params, d, genc, gparams, gx, aggregate);
+}
+
 // Cotangent g_gx of grad_x = sum_{l,f,c} genc[l,f] * scale_l * dw[c] . theta[c,f]:
 //   grad_grad_enc[l,f] = scale_l * sum_c (g_gx . dw[c]) theta[c,f]
 //   grad_params[c,f]  += scale_l * (g_gx . dw[c]) * genc[l,f]
@@ -181,8 +205,9 @@ hash_bwd_bwd_kernel(const float* __restrict__ x, int64_t n_cap,
                     HashGridDesc d, const float* __restrict__ genc,
                     const float* __restrict__ ggx, float* __restrict__ gge,
                     float* __restrict__ gparams, float* __restrict__ gx2) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= dev_count(n_cap, n_dev)) return;
+  const int64_t n = dev_count(n_cap, n_dev);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   const float v[3] = {ggx[3 * i], ggx[3 * i + 1], ggx[3 * i + 2]};
   float acc2[3] = {0.f, 0.f, 0.f};
@@ -240,6 +265,7 @@ hash_bwd_bwd_kernel(const float* __restrict__ x, int64_t n_cap,
 #pragma unroll
     for (int k = 0; k < 3; ++k) gx2[3 * i + k] = acc2[k];
   }
+  }
 }
 
 static int hg_setup(const char* who, int32_t n_levels, int32_t F, int32_t log2_T, int32_t base,
@@ -293,7 +319,7 @@ int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
   SN_REQUIRE(n_cap >= 0, "sn_hashgrid_fwd: n < 0");
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && enc, "sn_hashgrid_fwd: null pointer");
-  SN_DISPATCH_F(n_features, hash_fwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+  SN_DISPATCH_F(n_features, hash_fwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
                                                   as_stream(stream)>>>(x, n_cap, n_dev, params,
                                                                        d, enc));
   return check_launch("sn_hashgrid_fwd");
@@ -313,7 +339,7 @@ int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params16 && enc, "sn_hashgrid_fwd16: null pointer");
   const _Float16* p16 = reinterpret_cast<const _Float16*>(params16);
-  SN_DISPATCH_F(n_features, (hash_fwd_kernel<kF, _Float16><<<(unsigned)ceil_div(n_cap, 256), 256,
+  SN_DISPATCH_F(n_features, (hash_fwd_kernel<kF, _Float16><<<grid_for(n_cap, 256, kHashBlocks), 256,
                                                               0, as_stream(stream)>>>(
                                 x, n_cap, n_dev, p16, d, enc)));
   return check_launch("sn_hashgrid_fwd16");
@@ -340,7 +366,7 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
   SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
   SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
   SN_DISPATCH_F(n_features,
-                hash_bwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+                hash_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
                                       as_stream(stream)>>>(x, n_cap, n_dev, params, d, grad_enc,
                                                            grad_params, grad_x, g_aggregate));
   return check_launch("sn_hashgrid_bwd");
@@ -371,7 +397,7 @@ int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
   SN_REQUIRE(x && params && g_gx, "sn_hashgrid_bwd_bwd: null input");
   SN_REQUIRE(grad_grad_enc || ((grad_params || grad_x) && grad_enc),
              "sn_hashgrid_bwd_bwd: no output");
-  SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<(unsigned)ceil_div(n_cap, 256), 256, 0,
+  SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
                                                       as_stream(stream)>>>(
                                 x, n_cap, n_dev, params, d, grad_enc, g_gx, grad_grad_enc,
                                 grad_params, grad_x));

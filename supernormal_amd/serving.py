@@ -75,7 +75,7 @@ def hash_entries(n_levels, log2_T, base, scale):
     return tot
 
 
-def job_bytes(spec: JobSpec, max_samples_per_ray=128, n_levels=16, F=2, patch=9) -> int:
+def job_bytes(spec: JobSpec, max_samples_per_ray=1024, n_levels=16, F=2, patch=9) -> int:
     """Resident device bytes of a job: parameters with gradient and Adam
     moments, the step buffers at capacity, the dataset and the occupancy grid."""
     E = n_levels * F
@@ -106,7 +106,7 @@ class Job:
             ds = Dataset(scene, spec.n_views, spec.resolution, spec.resolution, 0.02,
                          self.device, seed=spec.seed, elevations=(0.0, 20.0, -20.0, 40.0, -40.0))
             conf = synthetic_conf("prims1024", end_iter=spec.max_iters,
-                                  batch_size=spec.batch_size, max_samples_per_ray=128,
+                                  batch_size=spec.batch_size, max_samples_per_ray=1024,
                                   mlp_precision=precision, step_mode="graph")
             conf["model.encoding"]["log2_hashmap_size"] = spec.log2_hashmap_size
             conf["model.ray_marching"]["occ_resolution"] = 128
