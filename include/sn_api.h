@@ -198,10 +198,12 @@ int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_le
                     float per_level_scale, int32_t n_active_levels, float* enc,
                     void* stream);
 /* dL/dparams += sum over points (atomic fp32 adds; caller zeroes).  grad_x  */
-/* (nullable) = dL/dx (overwritten).  Coarse levels (scale < 192) first sum */
-/* the lanes of a wavefront that hit the same table row, unless disabled by */
-/* sn_hashgrid_set_aggregation(0) (process-wide switch, for measurement).   */
-void sn_hashgrid_set_aggregation(int32_t on);
+/* (nullable) = dL/dx (overwritten).  Before the atomics, contributions to  */
+/* the same table row are merged: mode 2 (default, F <= 2) per workgroup in */
+/* an LDS hash table over all levels, mode 1 across a wavefront on coarse   */
+/* levels, mode 0 none.  sn_hashgrid_set_aggregation(mode) is a process-    */
+/* wide switch (no counterpart in the reference; for measurement and tests).*/
+void sn_hashgrid_set_aggregation(int32_t mode);
 int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,
                     int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
                     float per_level_scale, int32_t n_active_levels, const float* grad_enc,

@@ -193,6 +193,113 @@ hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
     hash_bwd_point<F>(i0, n, x, params, d, genc, gparams, gx, aggregate);
 }
 
+// Backward with the table gradient merged per workgroup in LDS first.  A
+// workgroup takes 256 consecutive points (a few samples of one patch's rays:
+// spatially compact) and, level by level, adds every corner contribution into
+// an LDS hash table keyed by the entry index (open addressing, kLdsSlots
+// slots, LDS atomics); the distinct entries are then flushed to the global
+// gradient with one fp32 atomic per feature.  Consecutive samples share most
+// corners on all but the finest levels, so the global atomics drop several-
+// fold; an insertion that finds no slot within kMaxProbe probes falls back to
+// a global atomic.  Same sums as the plain kernel, in another order.
+constexpr int kLdsSlots = 4096;
+constexpr int kMaxProbe = 32;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+template <int F>
+__global__ void __launch_bounds__(256)
+hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                    const float* __restrict__ params, HashGridDesc d,
+                    const float* __restrict__ genc, float* __restrict__ gparams,
+                    float* __restrict__ gx) {
+  __shared__ uint32_t keys[kLdsSlots];
+  __shared__ float vals[kLdsSlots * F];
+  const int tid = threadIdx.x;
+  for (int sl = tid; sl < kLdsSlots; sl += 256) {
+    keys[sl] = kEmpty;
+#pragma unroll
+    for (int f = 0; f < F; ++f) vals[sl * F + f] = 0.f;
+  }
+  __syncthreads();
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int64_t n_chunks = ceil_div(n, 256);
+  for (int64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {  // uniform per workgroup
+    const int64_t i = ch * 256 + tid;
+    const bool act = i < n;
+    float p[3] = {0.f, 0.f, 0.f};
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p[k] = x[3 * i + k];
+    }
+    const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
+    float dx[3] = {0.f, 0.f, 0.f};
+    for (int l = 0; l < d.n_active; ++l) {
+      float* tg = gparams + (int64_t)d.offset[l] * F;
+      if (act) {
+        float gl[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) gl[f] = g[l * F + f];
+        uint32_t idx[8];
+        float w[8], fr[3];
+        hg_level(p, d, l, idx, w, fr);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          uint32_t h = (idx[c] * 2654435761u) >> 20;  // 12 bits: kLdsSlots
+          int probe = 0;
+          for (;;) {
+            const uint32_t old = atomicCAS(&keys[h], kEmpty, idx[c]);
+            if (old == kEmpty || old == idx[c]) {
+#pragma unroll
+              for (int f = 0; f < F; ++f) atomicAdd(&vals[h * F + f], w[c] * gl[f]);
+              break;
+            }
+            if (++probe == kMaxProbe) {
+#pragma unroll
+              for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)idx[c] * F + f], w[c] * gl[f]);
+              break;
+            }
+            h = (h + 1) & (kLdsSlots - 1);
+          }
+        }
+        if (gx) {
+          float dw[8][3];
+          hg_dweights(fr, dw);
+          const float* tab = params + (int64_t)d.offset[l] * F;
+          float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            float sm = 0.f;
+#pragma unroll
+            for (int f = 0; f < F; ++f) sm += gl[f] * tab[(int64_t)idx[c] * F + f];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += dw[c][k] * sm;
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
+        }
+      }
+      __syncthreads();
+      // flush the merged entries of this level and clear the table
+      for (int sl = tid; sl < kLdsSlots; sl += 256) {
+        const uint32_t key = keys[sl];
+        if (key != kEmpty) {
+#pragma unroll
+          for (int f = 0; f < F; ++f) {
+            atomicAdd(&tg[(int64_t)key * F + f], vals[sl * F + f]);
+            vals[sl * F + f] = 0.f;
+          }
+          keys[sl] = kEmpty;
+        }
+      }
+      __syncthreads();
+    }
+    if (gx && act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gx[3 * i + k] = dx[k];
+    }
+  }
+}
+
 // Cotangent g_gx of grad_x = sum_{l,f,c} genc[l,f] * scale_l * dw[c] . theta[c,f]:
 //   grad_grad_enc[l,f] = scale_l * sum_c (g_gx . dw[c]) theta[c,f]
 //   grad_params[c,f]  += scale_l * (g_gx . dw[c]) * genc[l,f]
@@ -278,7 +385,9 @@ static int hg_setup(const char* who, int32_t n_levels, int32_t F, int32_t log2_T
   return SN_OK;
 }
 
-static int g_aggregate = 1;
+// 0: plain atomics, 1: wavefront aggregation of coarse levels, 2: workgroup
+// LDS merge of every level (F <= 2; wider features use 1)
+static int g_aggregate = 2;
 
 #define SN_DISPATCH_F(F, ...)          \
   switch (F) {                         \
@@ -294,7 +403,7 @@ using namespace sn;
 
 extern "C" {
 
-void sn_hashgrid_set_aggregation(int32_t on) { g_aggregate = on ? 1 : 0; }
+void sn_hashgrid_set_aggregation(int32_t mode) { g_aggregate = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 
 int64_t sn_hashgrid_layout(int32_t n_levels, int32_t log2_hashmap_size, int32_t base_resolution,
                            float per_level_scale, uint32_t* offsets) {
@@ -365,10 +474,20 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
   SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
+  if (g_aggregate == 2 && grad_params && n_features <= 2) {
+    if (n_features == 1)
+      hash_bwd_lds_kernel<1><<<grid_for(n_cap, 256, kHashBlocks), 256, 0, as_stream(stream)>>>(
+          x, n_cap, n_dev, params, d, grad_enc, grad_params, grad_x);
+    else
+      hash_bwd_lds_kernel<2><<<grid_for(n_cap, 256, kHashBlocks), 256, 0, as_stream(stream)>>>(
+          x, n_cap, n_dev, params, d, grad_enc, grad_params, grad_x);
+    return check_launch("sn_hashgrid_bwd");
+  }
+  const int agg = g_aggregate >= 1 ? 1 : 0;
   SN_DISPATCH_F(n_features,
                 hash_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
                                       as_stream(stream)>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                           grad_params, grad_x, g_aggregate));
+                                                           grad_params, grad_x, agg));
   return check_launch("sn_hashgrid_bwd");
 }
 
