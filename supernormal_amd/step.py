@@ -67,6 +67,9 @@ class SyncFreeStep:
         self.adam_t.fill_(int(flat.step_count))
         self.graph = None
         self._graph_levels = None
+        # optional (start, end) events around the hash-grid backward; recorded
+        # inside the graph when created with external=True
+        self.hash_bwd_events = None
         # a generator of its own (seeded) makes the sampled patches independent
         # of other work on the device, e.g. other jobs' steps in between
         self.gen = None
@@ -275,9 +278,14 @@ class SyncFreeStep:
                  ptr(mlp_d), cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.g_sdf),
                  ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
                  self.ws_mlp.numel(), s)
+        ev = self.hash_bwd_events
+        if ev is not None:
+            ev[0].record()
         call("sn_hashgrid_bwd_dev", ptr(self.pts), c1, ptr(self.ne),
              ptr(self.net.encoding.params), *self._hash_args(), ptr(self.g_enc),
              ptr(self.net.encoding.params.grad), None, s)
+        if ev is not None:
+            ev[1].record()
         # weight norm (packed weights -> weight_v, weight_g, bias) and inv_s -> variance
         torch.autograd.backward([mlp, inv_s], [self.g_mlp, self.losses[3:4]])
         # Adam with lr and step count on the device
