@@ -21,9 +21,10 @@ so `value` is the run-average iterations/s.  Beside it:
   * plain_gpu: the reference-structured path (torch autograd over the drop-in
     kernels, torch Linear MLP) timed from the same trained state;
   * roofline: the dominant kernel's algorithmic bytes per launch / its mean
-    duration, from HIP events on its launch stream inside the timed region (in
-    graph mode the events are nodes of the replayed graph; every 50th step is
-    read back, a sync that costs well under 1% of the run);
+    duration, from HIP events on its launch stream around every launch of the
+    timed region (in graph mode the iteration is three graphs with the
+    hash-grid backward alone in the middle one, so the events sit between
+    replays);
   * cpu_baseline: the CPU oracle (torch restatement of the reference
     iteration) on the same workload, a bounded number of iterations;
   * accuracy: Chamfer distance / F-score of the visible surface points of the
@@ -110,8 +111,6 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
-    ap.add_argument("--roofline-every", type=int, default=50,
-                    help="sync-free modes: read the roofline kernel's events every k-th step")
     ap.add_argument("--max-samples", type=int, default=None,
                     help="samples per centre ray capacity (default: the workload's)")
     a = ap.parse_args(argv)
@@ -136,13 +135,10 @@ def main(argv=None):
         runner.train_iteration(i)
     torch.cuda.synchronize()
     timer = _lib.time_calls(a.roofline_kernel) if sf is None else None
-    sampled = []  # (ms, n, n_active) of the hash backward in sampled iterations
     if sf is not None:
-        # events around the kernel, recorded in the graph (external events);
-        # every `roofline_every`-th iteration is read back after a sync
-        sf.hash_bwd_events = (torch.cuda.Event(enable_timing=True, external=True),
-                              torch.cuda.Event(enable_timing=True, external=True))
-        sf.graph = None  # re-capture with the events
+        # events around the hash-grid backward's replay (its own graph), read at the end
+        sf.time_hash_bwd = True
+        sf.hash_times = []
     samples = 0
     samples_dev = torch.zeros(1, dtype=torch.float64, device=dev)
     t0 = time.perf_counter()
@@ -153,10 +149,6 @@ def main(argv=None):
         runner.train_iteration(i)
         if sf is not None:
             samples_dev += sf.ns
-            if (i - a.warmup) % a.roofline_every == a.roofline_every - 1:
-                sf.hash_bwd_events[1].synchronize()
-                sampled.append((sf.hash_bwd_events[0].elapsed_time(sf.hash_bwd_events[1]),
-                                int(sf.ne), runner.sdf_network.active_levels()))
         else:
             samples += getattr(runner, "last_samples", 0)
     e1.record()
@@ -172,9 +164,11 @@ def main(argv=None):
         # (x, n, params, levels, F, log2 T, base, scale, n_active, ...)
         nbytes = [byte_fn(int(args[1]), int(args[8]), grid) for _, _, args in timer.records]
     else:
-        # sampled launches of the timed region (device count read back with them)
-        durs = [ms for ms, _, _ in sampled]
-        nbytes = [byte_fn(n, la, grid) for _, n, la in sampled]
+        # every launch of the timed region (device count copied next to it)
+        sf.time_hash_bwd = False
+        timed = sf.hash_bwd_timings()
+        durs = [ms for ms, _, _ in timed]
+        nbytes = [byte_fn(n, la, grid) for _, n, la in timed]
     ms_per_step = ms / max(a.steps, 1)
     value = 1000.0 / ms_per_step
     roof = None

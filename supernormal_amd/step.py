@@ -67,9 +67,10 @@ class SyncFreeStep:
         self.adam_t.fill_(int(flat.step_count))
         self.graph = None
         self._graph_levels = None
-        # optional (start, end) events around the hash-grid backward; recorded
-        # inside the graph when created with external=True
-        self.hash_bwd_events = None
+        # optional timing of the hash-grid backward (events between replays)
+        self.time_hash_bwd = False
+        self.hash_times = []
+        self.hash_counts = torch.zeros(100000, dtype=torch.int32, device=self.dev)
         # a generator of its own (seeded) makes the sampled patches independent
         # of other work on the device, e.g. other jobs' steps in between
         self.gen = None
@@ -217,6 +218,14 @@ class SyncFreeStep:
                  cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), None, s)
 
     def _iteration(self, inputs=None):
+        self._part_a(inputs)
+        self._part_hash()
+        self._part_adam()
+
+    def _part_a(self, inputs=None):
+        """Everything up to the hash-grid backward: sampling, marching, the
+        visibility pre-pass, SDF, render + losses, MLP backward and the weight-norm /
+        variance chain."""
         s = stream_of(self.dev)
         B, P, c0, c1 = self.B, self.P, self.c0, self.c1
         cfg = self.cfg
@@ -278,17 +287,18 @@ class SyncFreeStep:
                  ptr(mlp_d), cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.g_sdf),
                  ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
                  self.ws_mlp.numel(), s)
-        ev = self.hash_bwd_events
-        if ev is not None:
-            ev[0].record()
-        call("sn_hashgrid_bwd_dev", ptr(self.pts), c1, ptr(self.ne),
-             ptr(self.net.encoding.params), *self._hash_args(), ptr(self.g_enc),
-             ptr(self.net.encoding.params.grad), None, s)
-        if ev is not None:
-            ev[1].record()
         # weight norm (packed weights -> weight_v, weight_g, bias) and inv_s -> variance
         torch.autograd.backward([mlp, inv_s], [self.g_mlp, self.losses[3:4]])
-        # Adam with lr and step count on the device
+
+    def _part_hash(self):
+        """Hash-grid backward: atomics straight into the flat gradient buffer."""
+        call("sn_hashgrid_bwd_dev", ptr(self.pts), self.c1, ptr(self.ne),
+             ptr(self.net.encoding.params), *self._hash_args(), ptr(self.g_enc),
+             ptr(self.net.encoding.params.grad), None, stream_of(self.dev))
+
+    def _part_adam(self):
+        """Adam with the learning rate and step count on the device."""
+        s = stream_of(self.dev)
         self.adam_t += 1
         f = self.flat
         if self.table16 is not None:
@@ -310,17 +320,48 @@ class SyncFreeStep:
         self.step_size.fill_(float(step_size))
         # the optimizer's step count is the host's (other drivers may have stepped it)
         self.adam_t.fill_(int(self.flat.step_count))
+        timing = self.time_hash_bwd
         if not use_graph or inputs is not None:
-            self._iteration(inputs)
+            self._part_a(inputs)
+            e = self._hash_events() if timing else None
+            self._part_hash()
+            if e is not None:
+                e[1].record()
+            self._part_adam()
             self.flat.step_count += 1
             return self.losses
         levels = self.net.active_levels()
         if self.graph is None or self._graph_levels != levels:
             self._capture()
             self._graph_levels = levels
-        self.graph.replay()
+        # three graphs: the hash-grid backward alone in the middle, so that
+        # events between the replays time it
+        self.graph[0].replay()
+        e = self._hash_events() if timing else None
+        self.graph[1].replay()
+        if e is not None:
+            e[1].record()
+        self.graph[2].replay()
         self.flat.step_count += 1
         return self.losses
+
+    def _hash_events(self):
+        """Records the start event of a timed hash-grid backward and keeps the
+        device count and level count it ran with."""
+        e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        e[0].record()
+        k = len(self.hash_times)
+        if k < self.hash_counts.numel():
+            self.hash_counts[k:k + 1].copy_(self.ne)
+        self.hash_times.append((e[0], e[1], self.net.active_levels()))
+        return e
+
+    def hash_bwd_timings(self):
+        """[(ms, n points, active levels)] of the timed hash-grid backwards."""
+        torch.cuda.synchronize(self.dev)
+        counts = self.hash_counts[:len(self.hash_times)].tolist()
+        return [(a.elapsed_time(b), int(n), la)
+                for (a, b, la), n in zip(self.hash_times, counts)]
 
     def _capture(self):
         # one eager warm-up on a side stream (allocator pools, kernel attributes),
@@ -338,9 +379,13 @@ class SyncFreeStep:
             t.copy_(v)
         if gen_state is not None:
             self.gen.set_state(gen_state)
-        g = torch.cuda.CUDAGraph()
+        graphs = [torch.cuda.CUDAGraph() for _ in range(3)]
         if self.gen is not None:
-            g.register_generator_state(self.gen)
-        with torch.cuda.graph(g):
-            self._iteration()
-        self.graph = g
+            graphs[0].register_generator_state(self.gen)
+        with torch.cuda.graph(graphs[0]):
+            self._part_a()
+        with torch.cuda.graph(graphs[1]):
+            self._part_hash()
+        with torch.cuda.graph(graphs[2]):
+            self._part_adam()
+        self.graph = graphs
