@@ -220,7 +220,7 @@ def main(argv=None):
         runner.fused = True
         t = time.perf_counter()
         for i in range(a.plain_iters):
-            runner.train_iteration(it0 + 1 + a.plain_iters + i)
+            runner.train_iteration(it0 + 1 + a.plain_iters + i, ahead=i + 1 < a.plain_iters)
         torch.cuda.synchronize()
         fused_same = a.plain_iters / (time.perf_counter() - t)
         out["plain_gpu"] = {"value": round(plain, 2), "unit": "train iters/s",

@@ -138,8 +138,11 @@ class Runner:
     def occ_eval_fn(self):
         return self.fused_step.occ_eval_fn if self.fused else self.renderer.occ_eval_fn
 
-    def train_iteration(self, iter_i):
-        """One iteration; returns the loss tensor (or None when skipped)."""
+    def train_iteration(self, iter_i, ahead=None):
+        """One iteration; returns the loss tensor (or None when skipped).
+        In graph mode the next iteration's march may run ahead (see
+        _march_ahead_step); `ahead` False forbids it, True allows it past
+        end_iter (a caller that will run more iterations)."""
         r = self.renderer
         r.sampling_step_size = 10 ** (np.log10(self.start_step_size) - self.slop_step * iter_i)
         r.occupancy_grid.every_n_step(step=iter_i, occ_eval_fn=self.occ_eval_fn(),
@@ -147,8 +150,10 @@ class Runner:
         if self.iter_step % self.increase_bindwidth_every == 0:
             self.sdf_network.increase_bandwidth()
         if self.fused and self.sf_step is not None:
-            losses = self.sf_step.step(self.lr, r.sampling_step_size,
-                                       use_graph=self.step_mode == "graph")
+            graph = self.step_mode == "graph"
+            losses = self.sf_step.step(self.lr, r.sampling_step_size, use_graph=graph,
+                                       next_step_size=self._march_ahead_step(iter_i, ahead)
+                                       if graph else None)
             self.iter_step += 1
             self.update_learning_rate()
             return losses[0] + losses[1] + losses[2]
@@ -174,6 +179,19 @@ class Runner:
         self.iter_step += 1
         self.update_learning_rate()
         return loss
+
+    def _march_ahead_step(self, iter_i, ahead=None):
+        """The next iteration's step size when its march may run ahead of this
+        iteration's end: not before an occupancy update, a bandwidth increase,
+        a checkpoint or the end of training (each changes what the march reads
+        or should see the state between iterations)."""
+        if ahead is False:
+            return None
+        k = self.iter_step + 1  # iter_step once this iteration is done
+        if ((iter_i + 1) % self.occ_update_freq == 0 or k % self.increase_bindwidth_every == 0
+                or k % self.save_freq == 0 or (ahead is None and k >= self.end_iter)):
+            return None
+        return 10 ** (np.log10(self.start_step_size) - self.slop_step * (iter_i + 1))
 
     def _reference_structured_loss(self, o, d, plane_n, v_inv, n_true, mask, near, far):
         out = self.renderer.render(o, d, plane_n, near, far, v_inv,
@@ -258,6 +276,8 @@ class Runner:
             torch.cuda.set_rng_state(ck["rng_cuda"], self.device)
         if self.sf_step is not None:
             # new parameter buffers: new step buffers and graph
+            self.sf_step.drain()
+            torch.cuda.synchronize(self.device)
             self._make_sync_free_step()
 
 

@@ -159,6 +159,8 @@ class Job:
     def finish(self):
         """Mesh extraction; returns when the job's work is done."""
         self.mesh = self.extract_mesh()
+        with torch.cuda.stream(self.stream):
+            self.runner.sf_step.drain()
         self.stream.synchronize()
 
     def release(self):

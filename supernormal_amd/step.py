@@ -21,9 +21,16 @@ sequence of launches on one stream:
      norm and variance chain (autograd over the small parameter tensors)
   -> sn_adam_step with the learning rate and step count read from the device
 
-and can be captured once into a hipGraph (torch.cuda.CUDAGraph) and replayed.
-The occupancy-grid update (every n iterations) and the bandwidth increase run
-between replays; a change of the active hash levels re-captures the graph.
+and is captured once into four HIP graphs (torch.cuda.CUDAGraph): sampling +
+march, the rest of the forward and backward, the hash-grid backward, Adam.
+The march reads no parameter, so the next iteration's sampling + march can be
+replayed on a side stream as soon as this iteration's second graph is done,
+overlapping the hash-grid backward and Adam (which fill the device while the
+march, one lane per centre ray, occupies only a few compute units).  The
+occupancy-grid update (every n iterations) and the bandwidth increase run
+between replays (no march runs ahead across them); a change of the active hash
+levels re-captures the graphs.  While a march is pending, the buffers it writes
+(inputs, counts, packed0, n0, ri0/t00/t10) already hold the next iteration's.
 
 Only the 'dfd' gradient method is supported here (the analytic gradient uses
 FusedStep), and max_samples_per_ray must be set (it bounds the capacity).
@@ -67,6 +74,9 @@ class SyncFreeStep:
         self.adam_t.fill_(int(flat.step_count))
         self.graph = None
         self._graph_levels = None
+        # the next iteration's march when it ran ahead: (event, step size)
+        self._pending = None
+        self.side = None
         # optional timing of the hash-grid backward (events between replays)
         self.time_hash_bwd = False
         self.hash_times = []
@@ -218,40 +228,51 @@ class SyncFreeStep:
                  cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), None, s)
 
     def _iteration(self, inputs=None):
-        self._part_a(inputs)
+        self._part_march(inputs)
+        self._part_rest()
         self._part_hash()
         self._part_adam()
 
     def _part_a(self, inputs=None):
-        """Everything up to the hash-grid backward: sampling, marching, the
-        visibility pre-pass, SDF, render + losses, MLP backward and the weight-norm /
-        variance chain."""
+        self._part_march(inputs)
+        self._part_rest()
+
+    def _part_march(self, inputs=None):
+        """Patch sampling (or the given inputs) and the march of the centre rays.
+        Reads only the inputs, the step size and the occupancy grid, so it can run
+        ahead of the previous iteration's hash-grid backward and Adam step."""
         s = stream_of(self.dev)
-        B, P, c0, c1 = self.B, self.P, self.c0, self.c1
-        cfg = self.cfg
+        B = self.B
         grid = self.grid
         rx, ry, rz = (int(v) for v in grid.binary.shape)
-        self.flat.grad.zero_()
         if inputs is None:
             self._sample()
         elif not isinstance(inputs, str):  # "keep": the buffers hold the inputs already
             self._load(*inputs, stratified=False)
-        # weight-normed MLP weights and inv_s, differentiable w.r.t. the parameters
-        with torch.enable_grad():
-            mlp = self.net.packed_mlp()
-            inv_s = torch.exp(self.dev_net.variance * 10.0).clip(1e-6, 1e6).reshape(1)
-        self.inv_s.copy_(inv_s.detach())
-        mlp_d = mlp.detach()
-        roi = grid.roi_aabb
         # one march into per-ray slots, then packed (= count + fill of the reference)
         call("sn_ray_march_padded", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
-             ptr(self.t_max), B, ptr(roi), ptr(grid.binary), rx, ry, rz, 1.0,
+             ptr(self.t_max), B, ptr(grid.roi_aabb), ptr(grid.binary), rx, ry, rz, 1.0,
              ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
              ptr(self.t1_pad), s)
         call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
              ptr(self.ws_scan), self.ws_scan.numel(), s)
         call("sn_march_gather", ptr(self.packed0), B, self.msr, ptr(self.t0_pad),
              ptr(self.t1_pad), ptr(self.ri0), ptr(self.t00), ptr(self.t10), s)
+
+    def _part_rest(self):
+        """From the marched samples up to the hash-grid backward: the visibility
+        pre-pass, SDF, render + losses, MLP backward and the weight-norm / variance
+        chain."""
+        s = stream_of(self.dev)
+        B, c0, c1 = self.B, self.c0, self.c1
+        cfg = self.cfg
+        self.flat.grad.zero_()
+        # weight-normed MLP weights and inv_s, differentiable w.r.t. the parameters
+        with torch.enable_grad():
+            mlp = self.net.packed_mlp()
+            inv_s = torch.exp(self.dev_net.variance * 10.0).clip(1e-6, 1e6).reshape(1)
+        self.inv_s.copy_(inv_s.detach())
+        mlp_d = mlp.detach()
         # visibility pre-pass on the centre rays
         call("sn_centre_points", ptr(self.ri0), ptr(self.t00), ptr(self.t10), c0, ptr(self.n0),
              ptr(self.o_c), ptr(self.d_c), 3, ptr(self.didx0), ptr(self.nd0), ptr(self.pts0),
@@ -310,18 +331,26 @@ class SyncFreeStep:
                  f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0, ptr(self.adam_t), s)
 
     # ------------------------------------------------------------------- driver
-    def step(self, lr, step_size, use_graph=True, inputs=None):
+    def step(self, lr, step_size, use_graph=True, inputs=None, next_step_size=None):
         """One training iteration; lr / step_size for this iteration.  Returns
         the device tensor of losses (normal, mask, eikonal, dL/dinv_s).
         `inputs` = (rays_o, rays_d, plane_n, V_inv, normals, mask) of fixed
         patches with unjittered starts instead of sampling, or "keep" to rerun
-        on the inputs of the previous iteration (eager only)."""
+        on the inputs of the previous iteration (eager only).
+
+        Graph mode with `next_step_size`: the next iteration's sampling and march
+        are launched on a side stream once this iteration no longer reads their
+        buffers, so they overlap this iteration's hash-grid backward and Adam
+        step.  The caller must not change the occupancy grid before the next
+        step (the march has read it) and passes the same step size to it."""
         self.lr.fill_(float(lr))
-        self.step_size.fill_(float(step_size))
         # the optimizer's step count is the host's (other drivers may have stepped it)
         self.adam_t.fill_(int(self.flat.step_count))
         timing = self.time_hash_bwd
         if not use_graph or inputs is not None:
+            if self._pending is not None:
+                raise RuntimeError("an iteration's march is pending; run it in graph mode")
+            self.step_size.fill_(float(step_size))
             self._part_a(inputs)
             e = self._hash_events() if timing else None
             self._part_hash()
@@ -330,20 +359,62 @@ class SyncFreeStep:
             self._part_adam()
             self.flat.step_count += 1
             return self.losses
+        if self._pending is None:
+            self.step_size.fill_(float(step_size))
         levels = self.net.active_levels()
         if self.graph is None or self._graph_levels != levels:
             self._capture()
             self._graph_levels = levels
-        # three graphs: the hash-grid backward alone in the middle, so that
-        # events between the replays time it
-        self.graph[0].replay()
-        e = self._hash_events() if timing else None
+        cur = torch.cuda.current_stream(self.dev)
+        if self._pending is None:
+            self.graph[0].replay()
+        else:
+            ev, pending_step = self._pending
+            if float(step_size) != pending_step:
+                raise ValueError(f"step size {step_size} differs from the one the march "
+                                 f"ran ahead with ({pending_step})")
+            cur.wait_event(ev)
+            self._pending = None
         self.graph[1].replay()
+        if next_step_size is not None:
+            self._march_ahead(cur, float(next_step_size))
+        # the hash-grid backward alone in its graph, so that events between the
+        # replays time it
+        e = self._hash_events() if timing else None
+        self.graph[2].replay()
         if e is not None:
             e[1].record()
-        self.graph[2].replay()
+        self.graph[3].replay()
         self.flat.step_count += 1
         return self.losses
+
+    def _march_ahead(self, cur, step_size):
+        """Next iteration's sampling + march on the side stream, after this
+        iteration's last reader of those buffers (the graph just replayed on
+        `cur`)."""
+        if self.side is None:
+            self.side = torch.cuda.Stream(device=self.dev)
+        done = torch.cuda.Event()
+        done.record(cur)
+        self.side.wait_event(done)
+        with torch.cuda.stream(self.side):
+            self.step_size.fill_(step_size)
+            self.graph[0].replay()
+            ev = torch.cuda.Event()
+            ev.record(self.side)
+        self._pending = (ev, step_size)
+
+    @property
+    def march_pending(self):
+        return self._pending is not None
+
+    def drain(self):
+        """Discards a march that ran ahead once it has finished (the current
+        stream waits for it); call before freeing the step or changing what the
+        march read."""
+        if self._pending is not None:
+            torch.cuda.current_stream(self.dev).wait_event(self._pending[0])
+            self._pending = None
 
     def _hash_events(self):
         """Records the start event of a timed hash-grid backward and keeps the
@@ -365,27 +436,31 @@ class SyncFreeStep:
 
     def _capture(self):
         # one eager warm-up on a side stream (allocator pools, kernel attributes),
-        # then capture; the warm-up's update is undone from saved state
+        # then capture; the warm-up's update is undone from saved state.  A march
+        # that ran ahead keeps its inputs: the warm-up re-marches them ("keep").
         f = self.flat
+        torch.cuda.synchronize(self.dev)
         saved = [t.clone() for t in (f.data, f.exp_avg, f.exp_avg_sq, self.adam_t)]
         gen_state = self.gen.get_state() if self.gen is not None else None
         side = torch.cuda.Stream(device=self.dev)
         side.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(side):
-            self._iteration()
+            self._iteration("keep" if self._pending is not None else None)
         torch.cuda.current_stream(self.dev).wait_stream(side)
         torch.cuda.synchronize(self.dev)
         for t, v in zip((f.data, f.exp_avg, f.exp_avg_sq, self.adam_t), saved):
             t.copy_(v)
         if gen_state is not None:
             self.gen.set_state(gen_state)
-        graphs = [torch.cuda.CUDAGraph() for _ in range(3)]
+        graphs = [torch.cuda.CUDAGraph() for _ in range(4)]
         if self.gen is not None:
             graphs[0].register_generator_state(self.gen)
         with torch.cuda.graph(graphs[0]):
-            self._part_a()
+            self._part_march()
         with torch.cuda.graph(graphs[1]):
-            self._part_hash()
+            self._part_rest()
         with torch.cuda.graph(graphs[2]):
+            self._part_hash()
+        with torch.cuda.graph(graphs[3]):
             self._part_adam()
         self.graph = graphs

@@ -22,7 +22,7 @@ def _runner(dev, precision="fp32", step_mode="sync_free"):
     r.update_learning_rate()
     # a few ordinary iterations so that the occupancy grid and the field are not trivial
     for i in range(12):
-        r.train_iteration(i)
+        r.train_iteration(i, ahead=i < 11)  # no march left pending
     torch.cuda.synchronize()
     return r
 
@@ -94,3 +94,55 @@ def test_graph_replays_match_eager(dev):
         torch.testing.assert_close(losses_g, losses_e, rtol=1e-4, atol=1e-7)
         for a, b in zip(after_g, _state(flat)):
             torch.testing.assert_close(a, b, rtol=1e-4, atol=1e-6)
+
+
+def _seeded_runner(dev):
+    from supernormal_amd.runner import Runner, synthetic_conf
+    conf = synthetic_conf("sphere128", max_samples_per_ray=32, mlp_precision="fp16",
+                          step_mode="graph")
+    r = Runner(conf, fused=True, device=dev, seed=3, rng_seed=11)
+    r.update_learning_rate()
+    return r
+
+
+def test_march_ahead_matches_in_order(dev):
+    """Runs the same seeded training with the next iteration's march overlapped
+    (graph replays on a side stream) and strictly in order.  Sampling draws
+    from the run's own generator in the same order either way, so the samples
+    are the same and losses / counts agree up to the order of fp32 atomics.
+    Iterations 0..7 lie between two occupancy updates (0 and 8), so iterations
+    1..7 are marched ahead and nothing is pending after iteration 7."""
+    runs = []
+    for ahead in (None, False):
+        r = _seeded_runner(dev)
+        losses, counts = [], []
+        for i in range(8):
+            losses.append(r.train_iteration(i, ahead=ahead).clone())
+            counts.append(r.sf_step.ns.clone())
+            assert r.sf_step.march_pending == (ahead is None and i < 7)
+        torch.cuda.synchronize()
+        assert not r.sf_step.march_pending  # none past the last iteration
+        runs.append(([float(x) for x in losses], [int(x) for x in counts],
+                     r.optimizer.data.clone()))
+    (la, ca, pa), (lb, cb, pb) = runs
+    for a, b in zip(la, lb):
+        assert abs(a - b) <= 1e-3 * abs(b) + 1e-7
+    for a, b in zip(ca, cb):
+        assert abs(a - b) <= 1e-3 * b
+    scale = float(pb.abs().max())
+    assert float((pa - pb).abs().max()) <= 1e-3 * scale
+
+
+def test_march_ahead_step_size_must_match(dev):
+    r = _seeded_runner(dev)
+    sf = r.sf_step
+    r.update_learning_rate()
+    sf.step(r.lr, 5e-3, use_graph=True, next_step_size=4e-3)
+    assert sf.march_pending
+    with pytest.raises(ValueError):
+        sf.step(r.lr, 5e-3, use_graph=True)
+    with pytest.raises(RuntimeError):
+        sf.step(r.lr, 4e-3, use_graph=False)
+    sf.step(r.lr, 4e-3, use_graph=True)
+    torch.cuda.synchronize()
+    assert not sf.march_pending
