@@ -405,6 +405,25 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
                         int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
                         const float* grad_enc, float* grad_params, float* grad_x, void* stream);
+/* Backward as sn_hashgrid_bwd_dev with a workspace (F <= 2, mode 2), no   */
+/* fp32 atomics: corner contributions (summed across the lanes of a         */
+/* wavefront whose points share a cell) become records in per-bucket lists  */
+/* (8192 table rows per bucket); a second kernel sums each list as 64-bit   */
+/* fixed point scaled by the level's largest value and adds the sums to the */
+/* rows it owns.  Records beyond a list's capacity take fp32 atomics; with  */
+/* none, the result is bit-reproducible.  The capacity follows from          */
+/* workspace_bytes; sn_hashgrid_bwd_workspace_bytes gives the bytes for     */
+/* records_per_bucket (-1: unsupported config).  Same sums as               */
+/* sn_hashgrid_bwd_dev to fp32 rounding.                                     */
+int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
+                                        int32_t log2_hashmap_size, int32_t base_resolution,
+                                        float per_level_scale, int32_t records_per_bucket);
+int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                           const float* params, int32_t n_levels, int32_t n_features,
+                           int32_t log2_hashmap_size, int32_t base_resolution,
+                           float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                           float* grad_params, float* grad_x, void* workspace,
+                           size_t workspace_bytes, void* stream);
 int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                             const float* params, int32_t n_levels, int32_t n_features,
                             int32_t log2_hashmap_size, int32_t base_resolution,

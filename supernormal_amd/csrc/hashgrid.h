@@ -36,14 +36,15 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
 
 #ifdef __HIPCC__
 // Corner indices/weights of one level.  frac in [0,1); w[c] in corner order
-// c = bx | by<<1 | bz<<2, computed as ((w_x) * w_y) * w_z like the reference loop.
+// c = bx | by<<1 | bz<<2, computed as ((w_x) * w_y) * w_z like the reference loop;
+// g = the cell's integer coordinates (corner 0).
 __device__ __forceinline__ void hg_level(const float x[3], const HashGridDesc& d, int l,
-                                         uint32_t idx[8], float w[8], float frac[3]) {
+                                         uint32_t idx[8], float w[8], float frac[3],
+                                         uint32_t g[3]) {
   const float sc = d.scale[l];
   const uint32_t res = d.res[l];
   const uint32_t hs = d.hsize[l];
   const bool dense = (d.dense_mask >> l) & 1u;
-  uint32_t g[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float p = fmaf(sc, x[k], 0.5f);
@@ -67,6 +68,12 @@ __device__ __forceinline__ void hg_level(const float x[3], const HashGridDesc& d
     wt *= ((c >> 2) & 1) ? frac[2] : 1.f - frac[2];
     w[c] = wt;
   }
+}
+
+__device__ __forceinline__ void hg_level(const float x[3], const HashGridDesc& d, int l,
+                                         uint32_t idx[8], float w[8], float frac[3]) {
+  uint32_t g[3];
+  hg_level(x, d, l, idx, w, frac, g);
 }
 
 // d w[c] / d frac[k]

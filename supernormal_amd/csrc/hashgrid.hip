@@ -198,10 +198,11 @@ hash_bwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
 // spatially compact) and, level by level, adds every corner contribution into
 // an LDS hash table keyed by the entry index (open addressing, kLdsSlots
 // slots, LDS atomics); the distinct entries are then flushed to the global
-// gradient with one fp32 atomic per feature.  Consecutive samples share most
-// corners on all but the finest levels, so the global atomics drop several-
-// fold; an insertion that finds no slot within kMaxProbe probes falls back to
-// a global atomic.  Same sums as the plain kernel, in another order.
+// gradient with one fp32 atomic per feature.  An insertion that finds no slot
+// within kMaxProbe probes falls back to a global atomic.  Same sums as the
+// plain kernel, in another order.  (Used without a workspace; the bucket
+// path below is faster: fp32 atomics, in LDS and in global memory, are slow
+// on this GPU -- about 2e11 and 2e10 lane operations/s for the device.)
 constexpr int kLdsSlots = 4096;
 constexpr int kMaxProbe = 32;
 constexpr uint32_t kEmpty = 0xffffffffu;
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(256)
 hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                     const float* __restrict__ params, HashGridDesc d,
                     const float* __restrict__ genc, float* __restrict__ gparams,
-                    float* __restrict__ gx, int level_lo, int level_hi) {
+                    float* __restrict__ gx) {
   __shared__ uint32_t keys[kLdsSlots];
   __shared__ float vals[kLdsSlots * F];
   const int tid = threadIdx.x;
@@ -233,7 +234,7 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
     }
     const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
     float dx[3] = {0.f, 0.f, 0.f};
-    for (int l = level_lo; l < level_hi; ++l) {
+    for (int l = 0; l < d.n_active; ++l) {
       float* tg = gparams + (int64_t)d.offset[l] * F;
       if (act) {
         float gl[F];
@@ -298,6 +299,265 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       for (int k = 0; k < 3; ++k) gx[3 * i + k] = dx[k];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket path (with a workspace).  Two kernels, no fp32 atomics:
+//
+// 1. hash_bwd_rec_kernel: a workgroup takes 256 consecutive points and, per
+//    level, turns their corner contributions into records (table row, F
+//    values).  Lanes of a wavefront whose points lie in the same cell (common
+//    on coarse levels: a wavefront holds a few neighbouring samples of one
+//    patch) are summed first with cross-lane (DPP) reductions and leave one
+//    record per corner; the other lanes leave one record per corner each.
+//    Records go to the list of their bucket (kBucketEntries consecutive rows
+//    of the level): ranks within the workgroup come from integer LDS atomics,
+//    room in each list from one global atomic per (workgroup, level, bucket).
+//    Records beyond a list's capacity are added with fp32 atomics instead.
+//    The largest |value| of each level is kept (integer atomic max on the
+//    bits of a non-negative float).
+// 2. hash_bwd_bucket_sum_kernel: one workgroup per bucket sums its records in
+//    LDS as 64-bit fixed point (integer LDS atomics; scale 2^K with
+//    K = 61 - ceil(log2(count * max|value|)), so no sum can overflow and each
+//    record is rounded to count * max|value| * 2^-62 at worst, far below fp32
+//    resolution of anything but values ~2^-40 of the level's largest), then
+//    adds the sums to its rows, which no other workgroup touches.
+// Integer sums do not depend on the order, so the result is reproducible
+// (when no list overflows).
+constexpr int kBucketShift = 13;
+constexpr int kBucketEntries = 1 << kBucketShift;
+constexpr int kMaxBucketsPerLevel = 512;
+constexpr int kMinGroup = 4;  // smallest same-cell lane group summed across lanes
+
+// Buckets of each level: level l owns global buckets [base[l], base[l+1]).
+struct BucketDesc {
+  int32_t base[kMaxLevels + 1];
+};
+
+struct BucketPool {
+  int32_t* cursor;    // [n_buckets] records reserved per bucket (may exceed cap)
+  uint32_t* maxbits;  // [kMaxLevels] bits of the largest |record value| per level
+  uint16_t* row;      // [n_buckets * cap] row within the bucket
+  float* val;         // [n_buckets * cap * F]
+  int32_t cap;        // records per bucket
+};
+
+// DPP row shift / broadcast (gfx9 encodings: row_shr:n = 0x110 + n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source read 0.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf,
+                                                    true));
+}
+
+// Sum over the wavefront's 64 lanes, returned uniform (via lane 63).
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f<0x111>(v);
+  v += dpp_f<0x112>(v);
+  v += dpp_f<0x114>(v);
+  v += dpp_f<0x118>(v);
+  v += dpp_f<0x142, 0xa>(v);
+  v += dpp_f<0x143, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                    const float* __restrict__ params, HashGridDesc d,
+                    const float* __restrict__ genc, float* __restrict__ gparams,
+                    float* __restrict__ gx, BucketDesc bd, BucketPool pool) {
+  __shared__ int32_t hist[kMaxBucketsPerLevel];
+  __shared__ int32_t rbase[kMaxBucketsPerLevel];
+  __shared__ uint32_t lmax;
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int b = tid; b < kMaxBucketsPerLevel; b += 256) hist[b] = 0;
+  if (tid == 0) lmax = 0u;
+  __syncthreads();
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int64_t n_chunks = ceil_div(n, 256);
+  for (int64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {  // uniform per workgroup
+    const int64_t i = ch * 256 + tid;
+    const bool act = i < n;
+    float p[3] = {0.f, 0.f, 0.f};
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p[k] = x[3 * i + k];
+    }
+    const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
+    float dx[3] = {0.f, 0.f, 0.f};
+    for (int l = 0; l < d.n_active; ++l) {
+      float gl[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
+      uint32_t idx[8], cell[3];
+      float w[8], fr[3];
+      hg_level(p, d, l, idx, w, fr, cell);
+      // records of this lane: its own corners, a group's sums (group leader) or none
+      float rv[8][F];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int f = 0; f < F; ++f) rv[c][f] = w[c] * gl[f];
+      bool has = act;
+      uint64_t pend = __ballot(act);
+      while (pend) {  // uniform
+        const int ld = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t c0 = __builtin_amdgcn_readlane(cell[0], ld);
+        const uint32_t c1 = __builtin_amdgcn_readlane(cell[1], ld);
+        const uint32_t c2 = __builtin_amdgcn_readlane(cell[2], ld);
+        const bool in = ((pend >> lane) & 1ull) && cell[0] == c0 && cell[1] == c1 && cell[2] == c2;
+        const uint64_t m = __ballot(in);
+        if (__popcll(m) < kMinGroup) break;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int f = 0; f < F; ++f) {
+            const float sum = wave_sum(in ? w[c] * gl[f] : 0.f);
+            if (lane == ld) rv[c][f] = sum;
+          }
+        if (in && lane != ld) has = false;
+        pend &= ~m;
+      }
+      // ranks within the workgroup's share of each bucket
+      const int gb0 = bd.base[l], nb = bd.base[l + 1] - gb0;
+      int32_t rank[8];
+      float vmax = 0.f;
+      if (has) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          rank[c] = atomicAdd(&hist[idx[c] >> kBucketShift], 1);
+#pragma unroll
+          for (int f = 0; f < F; ++f) vmax = fmaxf(vmax, fabsf(rv[c][f]));
+        }
+        atomicMax(&lmax, __float_as_uint(vmax));
+      }
+      if (gx && act) {
+        float dw[8][3];
+        hg_dweights(fr, dw);
+        const float* tab = params + (int64_t)d.offset[l] * F;
+        float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          float sm = 0.f;
+#pragma unroll
+          for (int f = 0; f < F; ++f) sm += gl[f] * tab[(int64_t)idx[c] * F + f];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc[k] += dw[c][k] * sm;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
+      }
+      __syncthreads();
+      // room in each bucket's list: one global atomic per non-empty bucket
+      for (int b = tid; b < nb; b += 256) {
+        const int32_t c = hist[b];
+        if (c) {
+          rbase[b] = atomicAdd(&pool.cursor[gb0 + b], c);
+          hist[b] = 0;
+        }
+      }
+      if (tid == 0) {
+        atomicMax(&pool.maxbits[l], lmax);
+        lmax = 0u;
+      }
+      __syncthreads();
+      if (has) {
+        float* tg = gparams + (int64_t)d.offset[l] * F;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int b = (int)(idx[c] >> kBucketShift);
+          const int32_t r = rbase[b] + rank[c];
+          if (r < pool.cap) {
+            const int64_t at = (int64_t)(gb0 + b) * pool.cap + r;
+            pool.row[at] = (uint16_t)(idx[c] & (kBucketEntries - 1));
+#pragma unroll
+            for (int f = 0; f < F; ++f) pool.val[at * F + f] = rv[c][f];
+          } else {
+#pragma unroll
+            for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)idx[c] * F + f], rv[c][f]);
+          }
+        }
+      }
+    }
+    if (gx && act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gx[3 * i + k] = dx[k];
+    }
+  }
+}
+
+template <int F>
+__global__ void __launch_bounds__(1024)
+hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
+                           float* __restrict__ gparams) {
+  __shared__ unsigned long long acc[kBucketEntries * F];
+  const int gb = blockIdx.x, tid = threadIdx.x;
+  int l = 0;
+  while (l + 1 < d.n_levels && bd.base[l + 1] <= gb) ++l;
+  const int32_t nrec = min(pool.cursor[gb], pool.cap);
+  const float vmax = __uint_as_float(pool.maxbits[l]);
+  if (nrec <= 0 || vmax == 0.f) return;  // uniform per workgroup
+  const int64_t at0 = (int64_t)gb * pool.cap;
+  const int64_t e0 = (int64_t)(gb - bd.base[l]) * kBucketEntries;
+  const int64_t rows = min((int64_t)kBucketEntries, (int64_t)d.hsize[l] - e0);
+  float* tg = gparams + ((int64_t)d.offset[l] + e0) * F;
+  if (!(vmax <= 3.402823466e38f)) {
+    // an inf or NaN among the level's values: fp32 sums, so that they propagate
+    float* accf = reinterpret_cast<float*>(acc);
+    for (int j = tid; j < kBucketEntries * F; j += 1024) accf[j] = 0.f;
+    __syncthreads();
+    for (int r = tid; r < nrec; r += 1024) {
+      const int row = pool.row[at0 + r];
+#pragma unroll
+      for (int f = 0; f < F; ++f) atomicAdd(&accf[row * F + f], pool.val[(at0 + r) * F + f]);
+    }
+    __syncthreads();
+    for (int64_t j = tid; j < rows * F; j += 1024) tg[j] += accf[j];
+    return;
+  }
+  int e;
+  frexp((double)vmax * (double)nrec, &e);  // count * max|value| < 2^e
+  const int K = 61 - e;
+  const double scale = ldexp(1.0, K), inv = ldexp(1.0, -K);
+  for (int j = tid; j < kBucketEntries * F; j += 1024) acc[j] = 0ull;
+  __syncthreads();
+  for (int r = tid; r < nrec; r += 1024) {
+    const int row = pool.row[at0 + r];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const long long q = llrint((double)pool.val[(at0 + r) * F + f] * scale);
+      atomicAdd(&acc[row * F + f], (unsigned long long)q);
+    }
+  }
+  __syncthreads();
+  for (int64_t j = tid; j < rows * F; j += 1024) {
+    const long long s = (long long)acc[j];
+    if (s) tg[j] += (float)((double)s * inv);
+  }
+}
+
+// Buckets of every level (kMaxBucketsPerLevel per level at most); returns the
+// total or -1 when a level has more.
+static int64_t make_bucket_desc(const HashGridDesc& d, BucketDesc* bd) {
+  int64_t tot = 0;
+  for (int l = 0; l < d.n_levels; ++l) {
+    bd->base[l] = (int32_t)tot;
+    const int64_t nb = ((int64_t)d.hsize[l] + kBucketEntries - 1) / kBucketEntries;
+    if (nb > kMaxBucketsPerLevel) return -1;
+    tot += nb;
+  }
+  bd->base[d.n_levels] = (int32_t)tot;
+  return tot;
+}
+
+// workspace: cursor[n_buckets] | maxbits[kMaxLevels] | row[n_buckets*cap] | val[n_buckets*cap*F]
+static size_t bucket_head_bytes(int64_t n_buckets) {
+  return align_up(sizeof(int32_t) * (n_buckets + kMaxLevels), 256);
+}
+static size_t bucket_ws_bytes(int64_t n_buckets, int F, int64_t cap) {
+  return bucket_head_bytes(n_buckets) + align_up(sizeof(uint16_t) * n_buckets * cap, 256) +
+         align_up(sizeof(float) * F * n_buckets * cap, 256);
 }
 
 // Cotangent g_gx of grad_x = sum_{l,f,c} genc[l,f] * scale_l * dw[c] . theta[c,f]:
@@ -461,10 +721,26 @@ int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_le
                              base_resolution, per_level_scale, n_active_levels, enc, stream);
 }
 
-int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
-                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
-                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
-                        const float* grad_enc, float* grad_params, float* grad_x, void* stream) {
+int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
+                                        int32_t log2_hashmap_size, int32_t base_resolution,
+                                        float per_level_scale, int32_t records_per_bucket) {
+  HashGridDesc d;
+  BucketDesc bd;
+  if (records_per_bucket <= 0 || n_features <= 0 || n_features > 2 ||
+      make_hashgrid_desc(n_levels, log2_hashmap_size, base_resolution, per_level_scale,
+                         n_levels, &d) < 0)
+    return -1;
+  const int64_t nb = make_bucket_desc(d, &bd);
+  if (nb < 0) return -1;
+  return (int64_t)bucket_ws_bytes(nb, n_features, records_per_bucket);
+}
+
+int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                           const float* params, int32_t n_levels, int32_t n_features,
+                           int32_t log2_hashmap_size, int32_t base_resolution,
+                           float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                           float* grad_params, float* grad_x, void* workspace,
+                           size_t workspace_bytes, void* stream) {
   clear_error();
   HashGridDesc d;
   int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
@@ -474,24 +750,75 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
   SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
+  hipStream_t s = as_stream(stream);
+  const unsigned grid = grid_for(n_cap, 256, kHashBlocks);
+  BucketDesc bd{};
+  BucketPool pool{nullptr, nullptr, nullptr, 0};
   if (g_aggregate == 2 && grad_params && n_features <= 2) {
+    // bucket records when a workspace is given (and the 32-bit record
+    // counters cannot overflow), else atomics from the workgroup merge
+    BucketDesc bd{};
+    const int64_t nb = make_bucket_desc(d, &bd);
+    const bool buckets = workspace && nb > 0 && n_cap < (1ll << 28) - 4096 && d.n_active > 0;
+    if (buckets) {
+      const size_t head = bucket_head_bytes(nb);
+      const size_t per = (sizeof(uint16_t) + sizeof(float) * n_features) * (size_t)nb;
+      int64_t cap =
+          workspace_bytes > head + 512 ? (int64_t)((workspace_bytes - head - 512) / per) : 0;
+      if (cap > (1 << 24)) cap = 1 << 24;
+      while (cap > 0 && bucket_ws_bytes(nb, n_features, cap) > workspace_bytes) --cap;
+      if (cap <= 0) {
+        set_error("sn_hashgrid_bwd: workspace of %zu bytes holds no bucket records",
+                  workspace_bytes);
+        return SN_ESPACE;
+      }
+      char* w = static_cast<char*>(workspace);
+      BucketPool pool;
+      pool.cursor = reinterpret_cast<int32_t*>(w);
+      pool.maxbits = reinterpret_cast<uint32_t*>(w) + nb;
+      pool.row = reinterpret_cast<uint16_t*>(w + head);
+      pool.val = reinterpret_cast<float*>(w + head + align_up(sizeof(uint16_t) * nb * cap, 256));
+      pool.cap = (int32_t)cap;
+      const int32_t nb_active = bd.base[d.n_active];
+      rc = fill_i32(pool.cursor, nb + kMaxLevels, 0, s);  // cursors and level maxima
+      if (rc) return rc;
+      if (n_features == 1)
+        hash_bwd_rec_kernel<1><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
+                                                    grad_params, grad_x, bd, pool);
+      else
+        hash_bwd_rec_kernel<2><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
+                                                    grad_params, grad_x, bd, pool);
+      rc = check_launch("sn_hashgrid_bwd");
+      if (rc) return rc;
+      if (n_features == 1)
+        hash_bwd_bucket_sum_kernel<1><<<nb_active, 1024, 0, s>>>(d, bd, pool, grad_params);
+      else
+        hash_bwd_bucket_sum_kernel<2><<<nb_active, 1024, 0, s>>>(d, bd, pool, grad_params);
+      return check_launch("sn_hashgrid_bwd");
+    }
     // (one launch over all levels: measured faster than a launch per level)
-    const unsigned grid = grid_for(n_cap, 256, kHashBlocks);
-    hipStream_t s = as_stream(stream);
     if (n_features == 1)
       hash_bwd_lds_kernel<1><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                   grad_params, grad_x, 0, d.n_active);
+                                                  grad_params, grad_x);
     else
       hash_bwd_lds_kernel<2><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                   grad_params, grad_x, 0, d.n_active);
+                                                  grad_params, grad_x);
     return check_launch("sn_hashgrid_bwd");
   }
   const int agg = g_aggregate >= 1 ? 1 : 0;
   SN_DISPATCH_F(n_features,
-                hash_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
-                                      as_stream(stream)>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                           grad_params, grad_x, agg));
+                hash_bwd_kernel<kF><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
+                                                         grad_params, grad_x, agg));
   return check_launch("sn_hashgrid_bwd");
+}
+
+int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                        int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
+                        int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const float* grad_enc, float* grad_params, float* grad_x, void* stream) {
+  return sn_hashgrid_bwd_ws_dev(x, n_cap, n_dev, params, n_levels, n_features, log2_hashmap_size,
+                                base_resolution, per_level_scale, n_active_levels, grad_enc,
+                                grad_params, grad_x, nullptr, 0, stream);
 }
 
 int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_levels,

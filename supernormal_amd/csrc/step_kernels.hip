@@ -62,28 +62,44 @@ centre_points_kernel(const int64_t* __restrict__ ri, const float* __restrict__ t
 
 __device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + expf(-x)); }
 
-__global__ void __launch_bounds__(64)
+// One wavefront per ray: the lanes evaluate 64 consecutive samples' alphas at
+// once, then the transmittance runs as the sequential product T <- T (1 - a)
+// in sample order (each factor read from its sample's lane), so the flags are
+// those of a one-lane-per-ray loop, bit for bit.
+__global__ void __launch_bounds__(256)
 alpha_visibility_kernel(const int32_t* __restrict__ packed, int32_t n_rays,
                         const float* __restrict__ sdf, const int32_t* __restrict__ didx,
                         int64_t n_cap, const int32_t* __restrict__ n_dev,
                         const float* __restrict__ inv_s_dev, float eps,
                         uint8_t* __restrict__ keep) {
-  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  // the wavefront's ray (uniform: taken from its first lane)
+  const int64_t r = (int64_t)blockIdx.x * 4 + (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6);
   if (r >= n_rays) return;
   const int64_t n = dev_count(n_cap, n_dev);
   const int32_t base = packed[2 * r], cnt = packed[2 * r + 1];
   if (cnt <= 0 || base < 0 || (int64_t)base + cnt > n) return;
   const float inv_s = *inv_s_dev;
   float T = 1.f;
-  for (int32_t j = 0; j < cnt; ++j) {
-    const int64_t s = base + j;
-    const float fs = sdf[s];
-    const int32_t di = didx[s];
-    const float fe = di >= 0 ? sdf[n + di] : sdf[s + 1];
-    const float c = sigm(fs * inv_s), e = sigm(fe * inv_s);
-    const float a = fminf(fmaxf(((c - e) + 1e-5f) / (c + 1e-5f), 0.f), 1.f);
-    keep[s] = T >= eps ? 1 : 0;
-    T *= (1.f - a);
+  for (int32_t j0 = 0; j0 < cnt; j0 += 64) {
+    const int32_t j = j0 + lane;
+    float om = 1.f;
+    if (j < cnt) {
+      const int64_t s = base + j;
+      const float fs = sdf[s];
+      const int32_t di = didx[s];
+      const float fe = di >= 0 ? sdf[n + di] : sdf[s + 1];
+      const float c = sigm(fs * inv_s), e = sigm(fe * inv_s);
+      const float a = fminf(fmaxf(((c - e) + 1e-5f) / (c + 1e-5f), 0.f), 1.f);
+      om = 1.f - a;
+    }
+    const int m = min(64, cnt - j0);
+    float before = 1.f;
+    for (int k = 0; k < m; ++k) {
+      if (lane == k) before = T;
+      T *= __int_as_float(__builtin_amdgcn_readlane(__float_as_int(om), k));
+    }
+    if (j < cnt) keep[base + j] = before >= eps ? 1 : 0;
   }
 }
 
@@ -191,8 +207,9 @@ int sn_alpha_visibility(const int32_t* packed_info, int32_t n_rays, const float*
   SN_REQUIRE(n_rays >= 0 && n_cap >= 0, "sn_alpha_visibility: bad sizes");
   if (n_rays == 0 || n_cap == 0) return SN_OK;
   SN_REQUIRE(packed_info && sdf && diff_idx && inv_s && keep, "sn_alpha_visibility: null");
-  alpha_visibility_kernel<<<(unsigned)ceil_div(n_rays, 64), 64, 0, as_stream(stream)>>>(
-      packed_info, n_rays, sdf, diff_idx, n_cap, n_dev, inv_s, early_stop_eps, keep);
+  alpha_visibility_kernel<<<(unsigned)ceil_div((int64_t)n_rays * 64, 256), 256, 0,
+                            as_stream(stream)>>>(packed_info, n_rays, sdf, diff_idx, n_cap, n_dev,
+                                                 inv_s, early_stop_eps, keep);
   return check_launch("sn_alpha_visibility");
 }
 

@@ -20,6 +20,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of, workspace
+from .tcnn import hashgrid_bwd
 
 
 class SDFConfig:
@@ -105,8 +106,7 @@ class _SDFFn(torch.autograd.Function):
                  cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
                  ws.numel(), s)
         g_params = torch.zeros_like(params)
-        call("sn_hashgrid_bwd", ptr(x), n, ptr(params), *cfg.hash_args(ctx.n_active), ptr(g_enc),
-             ptr(g_params), None, s)
+        hashgrid_bwd(x, n, None, params, cfg.hash_args(ctx.n_active), g_enc, g_params, None, s)
         return None, g_params, g_mlp, None, None
 
 
@@ -148,8 +148,8 @@ class _SDFGradFn(torch.autograd.Function):
             call("sn_mlp_bwd", ptr(x), ptr(enc), cfg.E, n, None, ptr(mlp), cfg.d_hidden,
                  cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
                  ws.numel(), s)
-            call("sn_hashgrid_bwd", ptr(x), n, ptr(params), *cfg.hash_args(ctx.n_active),
-                 ptr(g_enc), ptr(g_params), None, s)
+            hashgrid_bwd(x, n, None, params, cfg.hash_args(ctx.n_active), g_enc, g_params,
+                         None, s)
             return None, g_params, g_mlp, None, None
         v = g_grad.contiguous()
         hargs = cfg.hash_args(ctx.n_active)
@@ -166,7 +166,7 @@ class _SDFGradFn(torch.autograd.Function):
              ptr(g_mlp), 0, ptr(ws), ws.numel(), s)
         ge = g_h0[:, 3:].contiguous()
         ged = g_h0d[:, 3:].contiguous()
-        call("sn_hashgrid_bwd", ptr(x), n, ptr(params), *hargs, ptr(ge), ptr(g_params), None, s)
+        hashgrid_bwd(x, n, None, params, hargs, ge, g_params, None, s)
         call("sn_hashgrid_bwd_bwd", ptr(x), n, ptr(params), *hargs, ptr(ged), ptr(v), None,
              ptr(g_params), None, s)
         return None, g_params, g_mlp, None, None

@@ -41,6 +41,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of
+from .tcnn import bucket_workspace_bytes, hashgrid_bwd
 
 
 class SyncFreeStep:
@@ -155,6 +156,8 @@ class SyncFreeStep:
         self.ws_scan = nb(lib.sn_scan_workspace_bytes(B))
         self.ws_step = nb(lib.sn_step_workspace_bytes(c0))
         self.ws_compact = nb(lib.sn_compact_workspace_bytes(c0))
+        hb = bucket_workspace_bytes(self.cfg.hash_args(self.cfg.grid["n_levels"]), self.c1)
+        self.ws_hash = nb(hb) if hb else None
         cfg = self.cfg
         if self.precision == "fp16":
             wsb = lib.sn_mlp16_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers)
@@ -312,10 +315,11 @@ class SyncFreeStep:
         torch.autograd.backward([mlp, inv_s], [self.g_mlp, self.losses[3:4]])
 
     def _part_hash(self):
-        """Hash-grid backward: atomics straight into the flat gradient buffer."""
-        call("sn_hashgrid_bwd_dev", ptr(self.pts), self.c1, ptr(self.ne),
-             ptr(self.net.encoding.params), *self._hash_args(), ptr(self.g_enc),
-             ptr(self.net.encoding.params.grad), None, stream_of(self.dev))
+        """Hash-grid backward into the flat gradient buffer (bucket records,
+        then per-bucket sums)."""
+        hashgrid_bwd(self.pts, self.c1, self.ne, self.net.encoding.params, self._hash_args(),
+                     self.g_enc, self.net.encoding.params.grad, None, stream_of(self.dev),
+                     ws=self.ws_hash)
 
     def _part_adam(self):
         """Adam with the learning rate and step count on the device."""

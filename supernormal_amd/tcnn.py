@@ -51,6 +51,37 @@ def _gargs(g, n_active):
             g["per_level_scale"], int(n_active))
 
 
+_REC_MIN, _REC_MAX, _WS_MAX = 4096, 131072, 2 << 30
+
+
+def bucket_workspace_bytes(gargs, n_points):
+    """Bytes of the bucket workspace of the hash-grid backward over up to
+    n_points points (0 when the config has no bucket path).  Records per
+    bucket: twice the mean count if every point had 8 distinct corners per
+    level spread over the finest level's buckets, within [4096, 131072] and at
+    most ~2 GiB in total; records beyond it take the atomic path."""
+    n_levels, F, log2T, base, pls = gargs[:5]
+    lib = _lib.lib()
+    one = lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, 1)
+    if one <= 0:
+        return 0
+    n_buckets = max(1, one // (2 + 4 * F))
+    per_level = max(1, ((1 << log2T) + 8191) // 8192)
+    cap = 16 * int(n_points) // per_level
+    cap = min(_REC_MAX, max(_REC_MIN, cap), max(1, _WS_MAX // (n_buckets * (2 + 4 * F))))
+    return int(lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, int(cap)))
+
+
+def hashgrid_bwd(x, n, n_dev, params, gargs, genc, gparams, gx, stream, ws=None):
+    """sn_hashgrid_bwd_ws_dev over n points (n_dev: device count or None) with a
+    bucket workspace (allocated here unless given)."""
+    if ws is None and gparams is not None:
+        nb = bucket_workspace_bytes(gargs, n)
+        ws = torch.empty(nb, dtype=torch.uint8, device=x.device) if nb else None
+    call("sn_hashgrid_bwd_ws_dev", ptr(x), n, ptr(n_dev), ptr(params), *gargs, ptr(genc),
+         ptr(gparams), ptr(gx), ptr(ws), 0 if ws is None else ws.numel(), stream)
+
+
 class _HashBackwardFn(torch.autograd.Function):
     """(dL/denc, x, params) -> (dL/dx, dL/dparams); differentiable in its first
     output (tcnn's bwd_bwd_input)."""
@@ -60,8 +91,8 @@ class _HashBackwardFn(torch.autograd.Function):
         genc = genc.contiguous()
         gparams = torch.zeros_like(params)
         gx = torch.empty_like(x) if want_x else None
-        call("sn_hashgrid_bwd", ptr(x), x.shape[0], ptr(params), *_gargs(g, n_active),
-             ptr(genc), ptr(gparams), ptr(gx), stream_of(x.device))
+        hashgrid_bwd(x, x.shape[0], None, params, _gargs(g, n_active), genc, gparams, gx,
+                     stream_of(x.device))
         ctx.save_for_backward(genc, x, params)
         ctx.g, ctx.n_active = g, n_active
         if gx is None:

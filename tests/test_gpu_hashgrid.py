@@ -97,6 +97,54 @@ def test_backward_aggregation_matches_plain_atomics(name, dev):
         torch.testing.assert_close(gxs[k], gxs[2], rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("records", [65536, 7])
+def test_bucket_backward_matches_atomics(records, dev):
+    """sn_hashgrid_bwd_ws_dev (bucket records, then per-bucket fixed-point
+    sums) equals the atomic path within fp32 reassociation, adding onto a
+    non-zero gradient and giving the same x-gradient, and repeats bit for bit;
+    with 7 records per bucket almost every record takes the overflow atomics.
+    Only the first n_dev of n_cap points count.  The workspace starts as
+    garbage and is reused by a second call."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=7)
+    g = torch.Generator().manual_seed(8)
+    centers = torch.rand(200, 3, generator=g) * 1.6 - 0.8
+    x = (centers[:, None, :] + torch.randn(200, 200, 3, generator=g) * 0.003).reshape(-1, 3)
+    x = x.contiguous().to(dev)
+    n_cap, n_use = x.shape[0], 35000
+    n_dev = torch.tensor([n_use], dtype=torch.int32, device=dev)
+    genc = torch.randn(n_cap, e.n_output_dims, generator=g).to(dev)
+    gargs = (16, 2, 19, 16, 1.38, 16)
+    lib = _lib.lib()
+    nb = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], records)
+    assert nb > 0
+    ws = torch.full((nb,), 255, dtype=torch.uint8, device=dev)
+    base = torch.randn(e.params.shape, generator=g).to(dev) * 1e-3
+    s = stream_of(dev)
+    g_ref, gx_ref = base.clone(), torch.empty_like(x)
+    call("sn_hashgrid_bwd_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs, ptr(genc),
+         ptr(g_ref), ptr(gx_ref), s)
+    outs = []
+    for _ in range(2):
+        g_b, gx_b = base.clone(), torch.empty_like(x)
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs,
+             ptr(genc), ptr(g_b), ptr(gx_b), ptr(ws), nb, s)
+        torch.cuda.synchronize()
+        scale = float((g_ref - base).abs().max())
+        torch.testing.assert_close(g_b, g_ref, rtol=1e-4, atol=1e-5 * scale)
+        torch.testing.assert_close(gx_b[:n_use], gx_ref[:n_use], rtol=1e-5, atol=1e-6)
+        outs.append(g_b)
+    if records >= 65536:  # no list overflows (35000 points: <= 4375 records per list)
+        # integer (fixed-point) sums: the same bits on every run
+        assert torch.equal(outs[0], outs[1])
+    # a workspace too small for one record per bucket is refused
+    with pytest.raises(RuntimeError):
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs,
+             ptr(genc), ptr(g_b), None, ptr(ws), 64, s)
+
+
 def test_double_backward_vs_fp64(dev):
     """L = sum(v * dL1/dx) with dL1/dx = autograd.grad(sum(g * enc), x, create_graph=True):
     d L / d params, d L / d g (through enc's upstream gradient) and d L / d x."""
