@@ -141,6 +141,20 @@ int sn_ray_march_padded(const float* rays_o, const float* rays_d, const float* t
                         float step_size, const float* step_dev, float cone_angle,
                         int32_t max_per_ray, int32_t* num_steps, float* t_starts_pad,
                         float* t_ends_pad, void* stream);
+/* As sn_ray_march_padded, reading occupancy from 4x4x4 brick masks (built  */
+/* by sn_occ_pack_bricks from `grid`; NULL: the byte grid): one 8-byte read */
+/* serves the 64 cells of a brick.  Same samples.  No counterpart in the    */
+/* reference (its grid is the byte array).                                  */
+int sn_ray_march_padded_bricks(const float* rays_o, const float* rays_d, const float* t_min,
+                               const float* t_max, int32_t n_rays, const float* roi_aabb,
+                               const uint8_t* grid, const uint64_t* bricks, int32_t rx,
+                               int32_t ry, int32_t rz, float step_size, const float* step_dev,
+                               float cone_angle, int32_t max_per_ray, int32_t* num_steps,
+                               float* t_starts_pad, float* t_ends_pad, void* stream);
+/* bricks[((x>>2)*(ry/4) + (y>>2))*(rz/4) + (z>>2)] bit (x&3)|(y&3)<<2|(z&3)<<4 */
+/* = grid[x][y][z] != 0; resolutions must be multiples of 4.                 */
+int sn_occ_pack_bricks(const uint8_t* binary, int32_t rx, int32_t ry, int32_t rz,
+                       uint64_t* bricks, void* stream);
 int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_ray,
                     const float* t_starts_pad, const float* t_ends_pad, int64_t* ray_indices,
                     float* t_starts, float* t_ends, void* stream);

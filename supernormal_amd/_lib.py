@@ -68,6 +68,9 @@ SIGNATURES = {
     "sn_render_step_set_impl": (None, [_i32]),
     "sn_ray_march_padded": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _f32,
                                       _p, _f32, _i32, _p, _p, _p, _p]),
+    "sn_ray_march_padded_bricks": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _p, _i32, _i32,
+                                             _i32, _f32, _p, _f32, _i32, _p, _p, _p, _p]),
+    "sn_occ_pack_bricks": (C.c_int, [_p, _i32, _i32, _i32, _p, _p]),
     "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
     "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),

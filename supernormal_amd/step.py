@@ -125,6 +125,13 @@ class SyncFreeStep:
         self.t10 = torch.zeros(c0, **f32)
         self.t0_pad = torch.zeros(c0, **f32)   # ray i's samples at i * max_samples_per_ray
         self.t1_pad = torch.zeros(c0, **f32)
+        # occupancy brick masks (4x4x4 cells per 64-bit word), rebuilt from the
+        # grid by each march
+        rx, ry, rz = (int(v) for v in self.grid.binary.shape)
+        self.bricks = None
+        if rx % 4 == 0 and ry % 4 == 0 and rz % 4 == 0:
+            self.bricks = torch.zeros((rx // 4) * (ry // 4) * (rz // 4), dtype=torch.int64,
+                                      device=dev)
         # visibility pre-pass
         self.didx0 = torch.zeros(c0, **i32)
         self.nd0 = torch.zeros(1, **i32)
@@ -252,10 +259,16 @@ class SyncFreeStep:
             self._sample()
         elif not isinstance(inputs, str):  # "keep": the buffers hold the inputs already
             self._load(*inputs, stratified=False)
-        # one march into per-ray slots, then packed (= count + fill of the reference)
-        call("sn_ray_march_padded", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
-             ptr(self.t_max), B, ptr(grid.roi_aabb), ptr(grid.binary), rx, ry, rz, 1.0,
-             ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
+        # one march into per-ray slots, then packed (= count + fill of the reference);
+        # occupancy read through 4x4x4 brick masks of the current grid
+        bricks = self.bricks
+        if bricks is not None:
+            if bricks.numel() * 64 != rx * ry * rz:
+                raise ValueError("the occupancy grid changed resolution")
+            call("sn_occ_pack_bricks", ptr(grid.binary), rx, ry, rz, ptr(bricks), s)
+        call("sn_ray_march_padded_bricks", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
+             ptr(self.t_max), B, ptr(grid.roi_aabb), ptr(grid.binary), ptr(bricks), rx, ry, rz,
+             1.0, ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
              ptr(self.t1_pad), s)
         call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
              ptr(self.ws_scan), self.ws_scan.numel(), s)

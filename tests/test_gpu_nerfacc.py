@@ -207,17 +207,25 @@ def test_ray_marching_bitexact(dev, res, step, cap):
     np.testing.assert_array_equal(t1.cpu().numpy(), t1_o)
 
 
-@pytest.mark.parametrize("res,step,cap", [(128, 1e-3, 128), (64, 2e-3, 128), (32, 5e-3, 7)])
-def test_ray_marching_padded_bitexact(dev, res, step, cap):
+ROIS = {"pow2": [-1, -1, -1, 1, 1, 1], "odd": [-1.05, -0.95, -1.1, 1.0, 1.15, 0.95]}
+
+
+@pytest.mark.parametrize("res,step,cap,roi_name,bricks",
+                         [(128, 1e-3, 128, "pow2", True), (128, 1e-3, 128, "pow2", False),
+                          (64, 2e-3, 128, "odd", True), (64, 2e-3, 128, "odd", False),
+                          (32, 5e-3, 7, "pow2", False), (32, 5e-3, 7, "odd", True)])
+def test_ray_marching_padded_bitexact(dev, res, step, cap, roi_name, bricks):
     """One-pass padded march + gather == the oracle's count + fill, with the
-    step size read from device memory."""
+    step size read from device memory; occupancy read from the byte grid or
+    from 4x4x4 brick masks (sn_occ_pack_bricks), with a roi whose spans are
+    powers of two (divisions done as exact multiplications) or not."""
     from supernormal_amd._lib import call, lib, ptr, stream_of
     rng = np.random.default_rng(res + cap + 1)
     n = 2048
     o, d, near, far = _cam_rays(rng, n)
     near = near + rng.random(n).astype(np.float32) * np.float32(step)
     grid = _scene_grid(res)
-    roi = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    roi = np.array(ROIS[roi_name], np.float32)
     od, dd, nd, fd, rd = (_t(a, dev) for a in (o, d, near, far, roi))
     gd = _t(grid.astype(np.uint8), dev)
     step_d = torch.tensor([step], dtype=torch.float32, device=dev)
@@ -225,8 +233,20 @@ def test_ray_marching_padded_bitexact(dev, res, step, cap):
     t0p = torch.full((n * cap,), -1.0, device=dev)
     t1p = torch.full((n * cap,), -1.0, device=dev)
     st = stream_of(dev)
-    call("sn_ray_march_padded", ptr(od), ptr(dd), ptr(nd), ptr(fd), n, ptr(rd), ptr(gd), res,
-         res, res, 1.0, ptr(step_d), 0.0, cap, ptr(counts), ptr(t0p), ptr(t1p), st)
+    if bricks:
+        bk = torch.empty((res // 4) ** 3, dtype=torch.int64, device=dev)
+        call("sn_occ_pack_bricks", ptr(gd), res, res, res, ptr(bk), st)
+        # every brick bit is its cell
+        bits = ((bk.cpu().numpy().view(np.uint64)[:, None] >> np.arange(64, dtype=np.uint64)) & 1)
+        b = bits.reshape(res // 4, res // 4, res // 4, 4, 4, 4)  # bx by bz | z y x
+        np.testing.assert_array_equal(b.transpose(0, 5, 1, 4, 2, 3).reshape(res, res, res),
+                                      grid.astype(np.uint64))
+        call("sn_ray_march_padded_bricks", ptr(od), ptr(dd), ptr(nd), ptr(fd), n, ptr(rd),
+             ptr(gd), ptr(bk), res, res, res, 1.0, ptr(step_d), 0.0, cap, ptr(counts), ptr(t0p),
+             ptr(t1p), st)
+    else:
+        call("sn_ray_march_padded", ptr(od), ptr(dd), ptr(nd), ptr(fd), n, ptr(rd), ptr(gd), res,
+             res, res, 1.0, ptr(step_d), 0.0, cap, ptr(counts), ptr(t0p), ptr(t1p), st)
     packed = torch.empty(n, 2, dtype=torch.int32, device=dev)
     total = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = torch.empty(lib().sn_scan_workspace_bytes(n), dtype=torch.uint8, device=dev)
