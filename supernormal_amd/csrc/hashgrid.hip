@@ -361,15 +361,27 @@ __device__ __forceinline__ float wave_sum(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Records of one level wait in LDS, ordered by bucket, before they are
+// written: a bucket's records from one workgroup are consecutive in its list,
+// so the writes of consecutive staging entries are consecutive in memory.
+struct StagedRecord {
+  uint32_t key;  // row within the level
+  int32_t at;    // index in the pool, or -1: add with atomics (list full)
+  float v[2];
+};
+
 template <int F>
 __global__ void __launch_bounds__(256)
 hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                     const float* __restrict__ params, HashGridDesc d,
                     const float* __restrict__ genc, float* __restrict__ gparams,
                     float* __restrict__ gx, BucketDesc bd, BucketPool pool) {
-  __shared__ int32_t hist[kMaxBucketsPerLevel];
-  __shared__ int32_t rbase[kMaxBucketsPerLevel];
+  __shared__ int32_t hist[kMaxBucketsPerLevel];   // records per bucket (this level)
+  __shared__ int32_t lofs[kMaxBucketsPerLevel];   // their first staging slot
+  __shared__ int32_t rbase[kMaxBucketsPerLevel];  // their first index in the list
+  __shared__ StagedRecord stage[256 * 8];
   __shared__ uint32_t lmax;
+  __shared__ int32_t n_staged;
   const int tid = threadIdx.x, lane = tid & 63;
   for (int b = tid; b < kMaxBucketsPerLevel; b += 256) hist[b] = 0;
   if (tid == 0) lmax = 0u;
@@ -449,13 +461,34 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
       }
       __syncthreads();
-      // room in each bucket's list: one global atomic per non-empty bucket
-      for (int b = tid; b < nb; b += 256) {
-        const int32_t c = hist[b];
-        if (c) {
-          rbase[b] = atomicAdd(&pool.cursor[gb0 + b], c);
-          hist[b] = 0;
+      // wavefront 0 alone touches the counts here: staging offsets (exclusive
+      // scan), room in each bucket's list (one global atomic per non-empty
+      // bucket), counts back to zero for the next level
+      if (tid < 64) {
+        const int per = (nb + 63) / 64;  // buckets per lane, in order
+        int sum = 0;
+        for (int q = 0; q < per; ++q) {
+          const int b = tid * per + q;
+          if (b < nb) sum += hist[b];
         }
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += y;
+        }
+        int run = incl - sum;
+        for (int q = 0; q < per; ++q) {
+          const int b = tid * per + q;
+          if (b < nb) {
+            const int32_t c = hist[b];
+            lofs[b] = run;
+            run += c;
+            if (c) rbase[b] = atomicAdd(&pool.cursor[gb0 + b], c);
+            hist[b] = 0;
+          }
+        }
+        if (tid == 63) n_staged = incl;
       }
       if (tid == 0) {
         atomicMax(&pool.maxbits[l], lmax);
@@ -463,19 +496,30 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       }
       __syncthreads();
       if (has) {
-        float* tg = gparams + (int64_t)d.offset[l] * F;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int b = (int)(idx[c] >> kBucketShift);
           const int32_t r = rbase[b] + rank[c];
-          if (r < pool.cap) {
-            const int64_t at = (int64_t)(gb0 + b) * pool.cap + r;
-            pool.row[at] = (uint16_t)(idx[c] & (kBucketEntries - 1));
+          StagedRecord& e = stage[lofs[b] + rank[c]];
+          e.key = idx[c];
+          e.at = r < pool.cap ? (gb0 + b) * pool.cap + r : -1;
 #pragma unroll
-            for (int f = 0; f < F; ++f) pool.val[at * F + f] = rv[c][f];
+          for (int f = 0; f < F; ++f) e.v[f] = rv[c][f];
+        }
+      }
+      __syncthreads();
+      {
+        float* tg = gparams + (int64_t)d.offset[l] * F;
+        const int ns = n_staged;
+        for (int q = tid; q < ns; q += 256) {
+          const StagedRecord e = stage[q];
+          if (e.at >= 0) {
+            pool.row[e.at] = (uint16_t)(e.key & (kBucketEntries - 1));
+#pragma unroll
+            for (int f = 0; f < F; ++f) pool.val[(int64_t)e.at * F + f] = e.v[f];
           } else {
 #pragma unroll
-            for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)idx[c] * F + f], rv[c][f]);
+            for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)e.key * F + f], e.v[f]);
           }
         }
       }
