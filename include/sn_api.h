@@ -141,6 +141,43 @@ int sn_ray_march_padded(const float* rays_o, const float* rays_d, const float* t
                         float step_size, const float* step_dev, float cone_angle,
                         int32_t max_per_ray, int32_t* num_steps, float* t_starts_pad,
                         float* t_ends_pad, void* stream);
+/* ------------------------------------------------------------------------ */
+/* Patch sampling: replaces Dataset.gen_random_patches + near_far_from_sphere */
+/* (models/dataset_loader.py:223-297) and the stratified start               */
+/* (renderer.py:69-71) for given random numbers: cx, cy (int64 [B], drawn as */
+/* the reference's torch.randint ranges), sel (int64 [B], index into         */
+/* train_idx), u ([B] in [0,1), nullable: no jitter) times *step_dev.        */
+/* Dataset tensors: normals [n,H,W,3], v_inv_all [n,H,W,3,3], masks [n,H,W], */
+/* k_inv [n,ks,ks] (k_stride ks = 3 or 4), pose [n,4,4].  mask_mode 1:     */
+/* mask = (m > 0.5), 0: ones.  Outputs                                       */
+/* as gen_random_patches (rays_o, rays_d, plane_n, v_inv, normals, mask      */
+/* [B, Ph*Pw]), mask_sum = sum(mask) + 1e-5 in a fixed order, the centre    */
+/* rays o_c / d_c and t_min / t_max.  Indices are clamped into range.       */
+int sn_patch_sample(const int64_t* cx, const int64_t* cy, const int64_t* sel, const float* u,
+                    const float* step_dev, const int64_t* train_idx, const float* normals,
+                    const float* v_inv_all, const float* masks, const float* k_inv,
+                    int32_t k_stride, const float* pose, int32_t n_img, int32_t n_train,
+                    int32_t H, int32_t W, int32_t B, int32_t patch_h, int32_t patch_w,
+                    int32_t mask_mode,
+                    float* rays_o, float* rays_d, float* plane_n, float* v_inv,
+                    float* normals_out, float* mask, float* mask_sum, float* o_c, float* d_c,
+                    float* t_min, float* t_max, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* MLP weight packing with weight norm: replaces the torch ops of            */
+/* torch.nn.utils.weight_norm (fields.py:69-70, W = g v / ||v|| per output   */
+/* row) + pack_mlp's concatenation, and SingleVarianceNetwork's              */
+/* inv_s = clip(exp(10 variance), 1e-6, 1e6) (fields.py:133-139).           */
+/* `layers` is a HOST array of 6 int64 per layer: offsets of v, g (-1: no    */
+/* weight norm, W = v) and b in `flat`, offset of the layer in `packed`,     */
+/* out, in.  variance / inv_s may be NULL.                                   */
+int sn_mlp_pack_fwd(const float* flat, int32_t n_layers, const int64_t* layers, float* packed,
+                    const float* variance, float* inv_s, void* stream);
+/* Backward of the packing: grad (same offsets as flat) += dL/dv, dL/dg,     */
+/* dL/db from g_packed, and *g_variance += dL/dinv_s * d inv_s/d variance.  */
+int sn_mlp_pack_bwd(const float* flat, int32_t n_layers, const int64_t* layers,
+                    const float* g_packed, float* grad, const float* variance,
+                    const float* g_inv_s, float* g_variance, void* stream);
 /* As sn_ray_march_padded, reading occupancy from 4x4x4 brick masks (built  */
 /* by sn_occ_pack_bricks from `grid`; NULL: the byte grid): one 8-byte read */
 /* serves the 64 cells of a brick.  Same samples.  No counterpart in the    */

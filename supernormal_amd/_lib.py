@@ -71,6 +71,11 @@ SIGNATURES = {
     "sn_ray_march_padded_bricks": (C.c_int, [_p, _p, _p, _p, _i32, _p, _p, _p, _i32, _i32,
                                              _i32, _f32, _p, _f32, _i32, _p, _p, _p, _p]),
     "sn_occ_pack_bricks": (C.c_int, [_p, _i32, _i32, _i32, _p, _p]),
+    "sn_mlp_pack_fwd": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p]),
+    "sn_patch_sample": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i32, _i32, _i32,
+                                  _i32, _i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p,
+                                  _p, _p, _p, _p]),
+    "sn_mlp_pack_bwd": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p]),
     "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
     "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),

@@ -27,7 +27,7 @@ LIB_NAME = "libsupernormal_amd.so"
 ARCH = "gfx950"
 
 # Files that restate integer/ordering-sensitive reference semantics.
-NO_CONTRACT = {"ray_marching.hip", "occgrid.hip", "render_weight.hip", "render_step.hip",
+NO_CONTRACT = {"ray_marching.hip", "occgrid.hip", "render_weight.hip", "render_step.hip", "sampler.hip",
                "step_kernels.hip"}
 
 COMMON_FLAGS = [

@@ -44,6 +44,33 @@ from ._lib import call, ptr, stream_of
 from .tcnn import bucket_workspace_bytes, hashgrid_bwd
 
 
+def flat_offset(flat, t):
+    """Offset of parameter tensor t in the flat parameter buffer."""
+    k = [i for i, q in enumerate(flat.params) if q is t]
+    if not k:
+        raise ValueError("a network tensor is not among the optimised parameters")
+    return flat.offsets[k[0]]
+
+
+def mlp_pack_table(net, flat):
+    """Host table (int64 [layers, 6]) of the SDF MLP's tensors in the flat
+    parameter buffer for sn_mlp_pack_fwd/bwd: offsets of v, g (-1 without
+    weight norm) and b, the layer's offset in the packed buffer, out, in."""
+    import numpy as np
+    rows, off = [], 0
+    for l in range(net.num_layers - 1):
+        lin = getattr(net, "lin" + str(l))
+        if net.weight_norm:
+            v, g = lin.weight_v, lin.weight_g
+        else:
+            v, g = lin.weight, None
+        out, inn = v.shape
+        rows.append([flat_offset(flat, v), -1 if g is None else flat_offset(flat, g),
+                     flat_offset(flat, lin.bias), off, out, inn])
+        off += out * inn + out
+    return np.ascontiguousarray(np.array(rows, dtype=np.int64))
+
+
 class SyncFreeStep:
     def __init__(self, sdf_network, deviation_network, occupancy_grid, flat, dataset,
                  batch_size, patch_size=3, max_samples_per_ray=128, normal_weight=1.0,
@@ -125,6 +152,18 @@ class SyncFreeStep:
         self.t10 = torch.zeros(c0, **f32)
         self.t0_pad = torch.zeros(c0, **f32)   # ray i's samples at i * max_samples_per_ray
         self.t1_pad = torch.zeros(c0, **f32)
+        # the dataset's tensors as the sampling kernel reads them (contiguous,
+        # fixed dtypes: torch.inverse returns a column-major K^-1)
+        ds = self.dataset
+        self._ds_tensors = None
+        if all(hasattr(ds, k) for k in ("normals", "V_inverse_all", "masks",
+                                        "intrinsics_all_inv", "pose_all", "_train_idx")):
+            self._ds_tensors = dict(
+                normals=ds.normals.float().contiguous(), v_inv=ds.V_inverse_all.float().contiguous(),
+                masks=ds.masks.float().contiguous(),
+                k_inv=ds.intrinsics_all_inv.float().contiguous(),
+                pose=ds.pose_all.float().contiguous(),
+                train_idx=ds._train_idx.to(torch.int64).contiguous())
         # occupancy brick masks (4x4x4 cells per 64-bit word), rebuilt from the
         # grid by each march
         rx, ry, rz = (int(v) for v in self.grid.binary.shape)
@@ -173,6 +212,8 @@ class SyncFreeStep:
         self.ws_mlp = nb(wsb)
         n_mlp = lib.sn_mlp_n_params(3 + E, cfg.d_hidden, cfg.n_layers)
         self.g_mlp = torch.zeros(n_mlp, **f32)
+        self.mlp_packed = torch.zeros(n_mlp, **f32)
+        self._pack_table(n_mlp)
         self.table16 = None
         if self.half_table:
             table = self.net.encoding.params
@@ -183,6 +224,15 @@ class SyncFreeStep:
             self.table_off = self.flat.offsets[k[0]]
             self.refresh_copies()
 
+    def _pack_table(self, n_mlp):
+        self.pack_layers = mlp_pack_table(self.net, self.flat)
+        if int((self.pack_layers[:, 4] * (self.pack_layers[:, 5] + 1)).sum()) != n_mlp:
+            raise ValueError("packed MLP size differs from the MLP kernels' layout")
+        self.n_pack = len(self.pack_layers)
+        ov = flat_offset(self.flat, self.dev_net.variance)
+        self.var_data = self.flat.data[ov:ov + 1]
+        self.var_grad = self.flat.grad[ov:ov + 1]
+
     def refresh_copies(self):
         """Re-derive the fp16 table copy from the parameters (after they were
         changed outside the step, e.g. a checkpoint load)."""
@@ -192,11 +242,29 @@ class SyncFreeStep:
 
     # ------------------------------------------------------------- one iteration
     def _sample(self):
-        """Patch sampling into the static input buffers (device RNG only)."""
+        """Patch sampling into the static input buffers (device RNG only): the
+        random numbers of gen_random_patches and the stratified start, drawn in
+        the same order, then one sampling kernel (sn_patch_sample)."""
         ds = self.dataset
-        o, d, pn, vi, nt, m = ds.gen_random_patches(self.B, patch_H=self.Ph, patch_W=self.Pw,
-                                                     generator=self.gen)
-        self._load(o, d, pn, vi, nt, m, stratified=True)
+        t = self._ds_tensors
+        if t is None:
+            o, d, pn, vi, nt, m = ds.gen_random_patches(self.B, patch_H=self.Ph,
+                                                         patch_W=self.Pw, generator=self.gen)
+            self._load(o, d, pn, vi, nt, m, stratified=True)
+            return
+        B, Ph, Pw, dev, g = self.B, self.Ph, self.Pw, self.dev, self.gen
+        cx = torch.randint(Pw // 2, ds.W - 1 - Pw // 2, (B,), device=dev, generator=g)
+        cy = torch.randint(Ph // 2, ds.H - 1 - Ph // 2, (B,), device=dev, generator=g)
+        sel = torch.randint(0, len(ds.train_images), (B,), device=dev, generator=g)
+        u = torch.rand((B,), generator=g, device=dev, dtype=torch.float32)
+        call("sn_patch_sample", ptr(cx), ptr(cy), ptr(sel), ptr(u), ptr(self.step_size),
+             ptr(t["train_idx"]), ptr(t["normals"]), ptr(t["v_inv"]), ptr(t["masks"]),
+             ptr(t["k_inv"]), int(t["k_inv"].shape[-1]), ptr(t["pose"]),
+             int(t["normals"].shape[0]), int(t["train_idx"].numel()), ds.H, ds.W, B, Ph, Pw,
+             1 if self.mask_w > 0 else 0,
+             ptr(self.rays_o), ptr(self.rays_d), ptr(self.plane_n), ptr(self.v_inv),
+             ptr(self.n_true), ptr(self.mask), ptr(self.mask_sum), ptr(self.o_c),
+             ptr(self.d_c), ptr(self.t_min), ptr(self.t_max), stream_of(dev))
 
     def _load(self, o, d, pn, vi, nt, m, stratified):
         ds = self.dataset
@@ -282,13 +350,12 @@ class SyncFreeStep:
         s = stream_of(self.dev)
         B, c0, c1 = self.B, self.c0, self.c1
         cfg = self.cfg
-        self.flat.grad.zero_()
-        # weight-normed MLP weights and inv_s, differentiable w.r.t. the parameters
-        with torch.enable_grad():
-            mlp = self.net.packed_mlp()
-            inv_s = torch.exp(self.dev_net.variance * 10.0).clip(1e-6, 1e6).reshape(1)
-        self.inv_s.copy_(inv_s.detach())
-        mlp_d = mlp.detach()
+        f = self.flat
+        f.grad.zero_()
+        # weight-normed MLP weights packed for the kernels, and inv_s
+        call("sn_mlp_pack_fwd", ptr(f.data), self.n_pack, self.pack_layers.ctypes.data,
+             ptr(self.mlp_packed), ptr(self.var_data), ptr(self.inv_s), s)
+        mlp_d = self.mlp_packed
         # visibility pre-pass on the centre rays
         call("sn_centre_points", ptr(self.ri0), ptr(self.t00), ptr(self.t10), c0, ptr(self.n0),
              ptr(self.o_c), ptr(self.d_c), 3, ptr(self.didx0), ptr(self.nd0), ptr(self.pts0),
@@ -325,7 +392,9 @@ class SyncFreeStep:
                  ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
                  self.ws_mlp.numel(), s)
         # weight norm (packed weights -> weight_v, weight_g, bias) and inv_s -> variance
-        torch.autograd.backward([mlp, inv_s], [self.g_mlp, self.losses[3:4]])
+        call("sn_mlp_pack_bwd", ptr(f.data), self.n_pack, self.pack_layers.ctypes.data,
+             ptr(self.g_mlp), ptr(f.grad), ptr(self.var_data), ptr(self.losses[3:4]),
+             ptr(self.var_grad), s)
 
     def _part_hash(self):
         """Hash-grid backward into the flat gradient buffer (bucket records,
@@ -393,8 +462,11 @@ class SyncFreeStep:
             cur.wait_event(ev)
             self._pending = None
         self.graph[1].replay()
+        done = None
         if next_step_size is not None:
-            self._march_ahead(cur, float(next_step_size))
+            # the last reader of the march's buffers was the graph just replayed
+            done = torch.cuda.Event()
+            done.record(cur)
         # the hash-grid backward alone in its graph, so that events between the
         # replays time it
         e = self._hash_events() if timing else None
@@ -402,17 +474,19 @@ class SyncFreeStep:
         if e is not None:
             e[1].record()
         self.graph[3].replay()
+        # submitted after this iteration's last graphs, so that their launch does
+        # not wait for the host to submit the (many-node) sampling graph
+        if done is not None:
+            self._march_ahead(done, float(next_step_size))
         self.flat.step_count += 1
         return self.losses
 
-    def _march_ahead(self, cur, step_size):
-        """Next iteration's sampling + march on the side stream, after this
-        iteration's last reader of those buffers (the graph just replayed on
-        `cur`)."""
+    def _march_ahead(self, done, step_size):
+        """Next iteration's sampling + march on the side stream, once `done`
+        (recorded after this iteration's last reader of those buffers) has
+        completed."""
         if self.side is None:
             self.side = torch.cuda.Stream(device=self.dev)
-        done = torch.cuda.Event()
-        done.record(cur)
         self.side.wait_event(done)
         with torch.cuda.stream(self.side):
             self.step_size.fill_(step_size)

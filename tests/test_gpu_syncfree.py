@@ -31,8 +31,37 @@ def _state(flat):
     return [t.clone() for t in (flat.data, flat.exp_avg, flat.exp_avg_sq)]
 
 
+def _device_packing(r, sf, monkeypatch):
+    """FusedStep with the packed MLP weights and inv_s of the device kernels
+    (sn_mlp_pack_fwd) in value, torch autograd for their backward: the dfd
+    normals difference SDF values of neighbouring points, which turns
+    one-ulp differences of the weights (another norm summation order) into
+    1e-3 relative differences of some gradients."""
+    from supernormal_amd._lib import call, ptr, stream_of
+    net = r.sdf_network
+    orig_pack, orig_inv = net.packed_mlp, r.fused_step.inv_s
+
+    def dev_values():
+        packed = torch.empty_like(sf.mlp_packed)
+        inv = torch.empty(1, device=sf.dev)
+        call("sn_mlp_pack_fwd", ptr(sf.flat.data), sf.n_pack, sf.pack_layers.ctypes.data,
+             ptr(packed), ptr(sf.var_data), ptr(inv), stream_of(sf.dev))
+        return packed, inv
+
+    def packed_mlp():
+        m = orig_pack()
+        return m + (dev_values()[0] - m).detach()
+
+    def inv_s():
+        t = orig_inv()
+        return t + (dev_values()[1] - t).detach()
+
+    monkeypatch.setattr(net, "packed_mlp", packed_mlp)
+    monkeypatch.setattr(r.fused_step, "inv_s", inv_s)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
-def test_sync_free_matches_fused_step(dev, precision):
+def test_sync_free_matches_fused_step(dev, precision, monkeypatch):
     r = _runner(dev, precision)
     fs, flat = r.fused_step, r.optimizer
     # FusedStep gathers from the fp32 table: compare without the fp16 table copy
@@ -41,6 +70,7 @@ def test_sync_free_matches_fused_step(dev, precision):
                       r.dataset, r.batch_size, r.patch_size, r.max_samples_per_ray,
                       r.normal_weight, r.eikonal_weight, r.mask_weight, r.loss_type,
                       precision=precision, device=dev, half_table=False)
+    _device_packing(r, sf, monkeypatch)
     g = torch.Generator(device=dev).manual_seed(7)
     o, d, pn, vi, nt, m = r.dataset.gen_random_patches(r.batch_size, generator=g)
     near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
