@@ -111,6 +111,10 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="after the timed run: 5 hash-grid backward calls on the last "
+                         "iteration's points and 3 copies of 64 MiB (calibration), for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--max-samples", type=int, default=None,
                     help="samples per centre ray capacity (default: the workload's)")
     a = ap.parse_args(argv)
@@ -205,6 +209,18 @@ def main(argv=None):
         "wall_s": round(wall, 2),
         "roofline": roof,
     }
+    if a.traffic_probe and sf is not None:
+        sf.drain()
+        torch.cuda.synchronize()
+        for _ in range(5):
+            sf._part_hash()
+        torch.cuda.synchronize()
+        xc = torch.rand(16 << 20, device=dev)
+        yc = torch.empty_like(xc)
+        for _ in range(3):
+            torch.mul(xc, 1.0, out=yc)
+        torch.cuda.synchronize()
+        out["traffic_probe"] = {"hash_bwd_points": int(sf.ne), "copy_bytes_each_way": 64 << 20}
     if not a.no_accuracy:
         from supernormal_amd.mesh import chamfer_distance_and_f1_score, visible_surface_points
         net = runner.sdf_network

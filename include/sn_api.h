@@ -444,17 +444,23 @@ int sn_patch_points(const int64_t* ray_indices, const float* t_starts, const flo
 int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                           const void* params16, int32_t n_levels, int32_t n_features,
                           int32_t log2_hashmap_size, int32_t base_resolution,
-                          float per_level_scale, int32_t n_active_levels, float* enc,
+                          float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev, float* enc,
                           void* stream);
 /* Hash-grid encode / backward / double backward over the first *n_dev      */
 /* points of an n_cap-point buffer (tcnn Encoding, as sn_hashgrid_*).       */
+/* n_active_dev (nullable): device copy of the active level count, at most  */
+/* n_active_levels, read by the kernels, so that a captured launch follows  */
+/* the bandwidth schedule without re-capture.                                */
 int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
                         int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                         float* enc, void* stream);
 int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
                         int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                         const float* grad_enc, float* grad_params, float* grad_x, void* stream);
 /* Backward as sn_hashgrid_bwd_dev with a workspace (F <= 2, mode 2), no   */
 /* fp32 atomics: corner contributions (summed across the lanes of a         */
@@ -472,13 +478,15 @@ int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
 int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                            const float* params, int32_t n_levels, int32_t n_features,
                            int32_t log2_hashmap_size, int32_t base_resolution,
-                           float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                           float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev, const float* grad_enc,
                            float* grad_params, float* grad_x, void* workspace,
                            size_t workspace_bytes, void* stream);
 int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                             const float* params, int32_t n_levels, int32_t n_features,
                             int32_t log2_hashmap_size, int32_t base_resolution,
                             float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                             const float* grad_enc, const float* g_gx, float* grad_grad_enc,
                             float* grad_params, float* grad_x, void* stream);
 /* Cells of an occupancy update after warm-up (NA/nerfacc/grid.py:181-194): */

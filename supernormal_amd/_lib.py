@@ -82,8 +82,7 @@ SIGNATURES = {
     "sn_adam_step_copy16": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
                                       C.c_double, _i32, _p, _p, _i64, _i64, _p]),
     "sn_to_half": (C.c_int, [_p, _p, _i64, _p]),
-    "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
-                                        _p, _p]),
+    "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
                                          _p, _p, _sz, _p]),
@@ -93,15 +92,11 @@ SIGNATURES = {
     "sn_alpha_visibility": (C.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _f32, _p, _p]),
     "sn_patch_points": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _p, _i32, _i32, _p, _p, _p,
                                   _p, _p, _sz, _p]),
-    "sn_hashgrid_fwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p,
-                                      _p]),
-    "sn_hashgrid_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p,
-                                      _p, _p, _p]),
+    "sn_hashgrid_fwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
+    "sn_hashgrid_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p]),
     "sn_hashgrid_bwd_workspace_bytes": (_i64, [_i32, _i32, _i32, _i32, _f32, _i32]),
-    "sn_hashgrid_bwd_ws_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
-                                         _p, _p, _p, _p, _sz, _p]),
-    "sn_hashgrid_bwd_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
-                                          _p, _p, _p, _p, _p, _p]),
+    "sn_hashgrid_bwd_ws_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _sz, _p]),
+    "sn_hashgrid_bwd_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _p, _p]),
     "sn_occ_sample_workspace_bytes": (_sz, [_i32]),
     "sn_occ_sample_cells": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _sz, _p]),
 }

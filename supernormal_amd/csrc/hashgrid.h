@@ -23,6 +23,9 @@ constexpr int kMaxLevels = 32;
 struct HashGridDesc {
   int32_t n_levels;
   int32_t n_active;  // levels >= n_active are masked (output 0, no gradient)
+  // optional device copy of the active level count (read by the kernels,
+  // bounded by n_active): lets a captured launch follow a changing count
+  const int32_t* n_active_dev;
   uint32_t offset[kMaxLevels + 1];  // table entry offsets (not floats)
   uint32_t hsize[kMaxLevels];
   uint32_t res[kMaxLevels];
@@ -74,6 +77,13 @@ __device__ __forceinline__ void hg_level(const float x[3], const HashGridDesc& d
                                          uint32_t idx[8], float w[8], float frac[3]) {
   uint32_t g[3];
   hg_level(x, d, l, idx, w, frac, g);
+}
+
+// Active levels seen by a kernel.
+__device__ __forceinline__ int active_levels(const HashGridDesc& d) {
+  if (!d.n_active_dev) return d.n_active;
+  const int v = *d.n_active_dev;
+  return v < 0 ? 0 : (v < d.n_active ? v : d.n_active);
 }
 
 // d w[c] / d frac[k]

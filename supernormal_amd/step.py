@@ -27,9 +27,9 @@ The march reads no parameter, so the next iteration's sampling + march can be
 replayed on a side stream as soon as this iteration's second graph is done,
 overlapping the hash-grid backward and Adam (which fill the device while the
 march, one lane per centre ray, occupies only a few compute units).  The
-occupancy-grid update (every n iterations) and the bandwidth increase run
-between replays (no march runs ahead across them); a change of the active hash
-levels re-captures the graphs.  While a march is pending, the buffers it writes
+occupancy-grid update (every n iterations) runs between replays (no march
+runs ahead across it); the active hash levels are a device scalar, so a
+bandwidth increase changes no launch.  While a march is pending, the buffers it writes
 (inputs, counts, packed0, n0, ri0/t00/t10) already hold the next iteration's.
 
 Only the 'dfd' gradient method is supported here (the analytic gradient uses
@@ -101,7 +101,6 @@ class SyncFreeStep:
         self._alloc()
         self.adam_t.fill_(int(flat.step_count))
         self.graph = None
-        self._graph_levels = None
         # the next iteration's march when it ran ahead: (event, step size)
         self._pending = None
         self.side = None
@@ -142,6 +141,7 @@ class SyncFreeStep:
         self.step_size = torch.zeros(1, **f32)
         self.lr = torch.zeros(1, **f32)
         self.adam_t = torch.zeros(1, **i32)
+        self.levels_dev = torch.zeros(1, **i32)  # active hash levels (bandwidth)
         self.inv_s = torch.zeros(1, **f32)
         # marching
         self.counts = torch.zeros(B, **i32)
@@ -164,13 +164,6 @@ class SyncFreeStep:
                 k_inv=ds.intrinsics_all_inv.float().contiguous(),
                 pose=ds.pose_all.float().contiguous(),
                 train_idx=ds._train_idx.to(torch.int64).contiguous())
-        # occupancy brick masks (4x4x4 cells per 64-bit word), rebuilt from the
-        # grid by each march
-        rx, ry, rz = (int(v) for v in self.grid.binary.shape)
-        self.bricks = None
-        if rx % 4 == 0 and ry % 4 == 0 and rz % 4 == 0:
-            self.bricks = torch.zeros((rx // 4) * (ry // 4) * (rz // 4), dtype=torch.int64,
-                                      device=dev)
         # visibility pre-pass
         self.didx0 = torch.zeros(c0, **i32)
         self.nd0 = torch.zeros(1, **i32)
@@ -288,16 +281,20 @@ class SyncFreeStep:
         self.t_max.copy_(far)
 
     def _hash_args(self):
-        return self.cfg.hash_args(self.net.active_levels())
+        """Hash-grid arguments with every level as the bound; the active count
+        is read from self.levels_dev (set by step()), so a bandwidth increase
+        changes no launch and needs no re-capture."""
+        return self.cfg.hash_args(self.cfg.grid["n_levels"])
 
     def _sdf_fwd(self, pts, enc, sdf, cap, n_dev, mlp, s):
         cfg = self.cfg
         if self.table16 is not None:
             call("sn_hashgrid_fwd16_dev", ptr(pts), cap, ptr(n_dev), ptr(self.table16),
-                 *self._hash_args(), ptr(enc), s)
+                 *self._hash_args(), ptr(self.levels_dev), ptr(enc), s)
         else:
             call("sn_hashgrid_fwd_dev", ptr(pts), cap, ptr(n_dev),
-                 ptr(self.net.encoding.params), *self._hash_args(), ptr(enc), s)
+                 ptr(self.net.encoding.params), *self._hash_args(), ptr(self.levels_dev),
+                 ptr(enc), s)
         if self.precision == "fp16":
             call("sn_mlp16_fwd", ptr(pts), ptr(enc), cfg.E, cap, ptr(n_dev), ptr(mlp),
                  cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(sdf), s)
@@ -328,15 +325,11 @@ class SyncFreeStep:
         elif not isinstance(inputs, str):  # "keep": the buffers hold the inputs already
             self._load(*inputs, stratified=False)
         # one march into per-ray slots, then packed (= count + fill of the reference);
-        # occupancy read through 4x4x4 brick masks of the current grid
-        bricks = self.bricks
-        if bricks is not None:
-            if bricks.numel() * 64 != rx * ry * rz:
-                raise ValueError("the occupancy grid changed resolution")
-            call("sn_occ_pack_bricks", ptr(grid.binary), rx, ry, rz, ptr(bricks), s)
-        call("sn_ray_march_padded_bricks", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
-             ptr(self.t_max), B, ptr(grid.roi_aabb), ptr(grid.binary), ptr(bricks), rx, ry, rz,
-             1.0, ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
+        # (the byte grid: brick masks, sn_ray_march_padded_bricks, measured no faster
+        # with the wavefront-per-ray march)
+        call("sn_ray_march_padded", ptr(self.o_c), ptr(self.d_c), ptr(self.t_min),
+             ptr(self.t_max), B, ptr(grid.roi_aabb), ptr(grid.binary), rx, ry, rz, 1.0,
+             ptr(self.step_size), 0.0, self.msr, ptr(self.counts), ptr(self.t0_pad),
              ptr(self.t1_pad), s)
         call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
              ptr(self.ws_scan), self.ws_scan.numel(), s)
@@ -401,7 +394,7 @@ class SyncFreeStep:
         then per-bucket sums)."""
         hashgrid_bwd(self.pts, self.c1, self.ne, self.net.encoding.params, self._hash_args(),
                      self.g_enc, self.net.encoding.params.grad, None, stream_of(self.dev),
-                     ws=self.ws_hash)
+                     ws=self.ws_hash, n_active_dev=self.levels_dev)
 
     def _part_adam(self):
         """Adam with the learning rate and step count on the device."""
@@ -430,6 +423,7 @@ class SyncFreeStep:
         step.  The caller must not change the occupancy grid before the next
         step (the march has read it) and passes the same step size to it."""
         self.lr.fill_(float(lr))
+        self.levels_dev.fill_(int(self.net.active_levels()))
         # the optimizer's step count is the host's (other drivers may have stepped it)
         self.adam_t.fill_(int(self.flat.step_count))
         timing = self.time_hash_bwd
@@ -447,10 +441,8 @@ class SyncFreeStep:
             return self.losses
         if self._pending is None:
             self.step_size.fill_(float(step_size))
-        levels = self.net.active_levels()
-        if self.graph is None or self._graph_levels != levels:
+        if self.graph is None:
             self._capture()
-            self._graph_levels = levels
         cur = torch.cuda.current_stream(self.dev)
         if self._pending is None:
             self.graph[0].replay()

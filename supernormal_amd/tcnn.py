@@ -72,14 +72,17 @@ def bucket_workspace_bytes(gargs, n_points):
     return int(lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, int(cap)))
 
 
-def hashgrid_bwd(x, n, n_dev, params, gargs, genc, gparams, gx, stream, ws=None):
+def hashgrid_bwd(x, n, n_dev, params, gargs, genc, gparams, gx, stream, ws=None,
+                 n_active_dev=None):
     """sn_hashgrid_bwd_ws_dev over n points (n_dev: device count or None) with a
-    bucket workspace (allocated here unless given)."""
+    bucket workspace (allocated here unless given); n_active_dev: device copy
+    of the active level count (or None: gargs' count)."""
     if ws is None and gparams is not None:
         nb = bucket_workspace_bytes(gargs, n)
         ws = torch.empty(nb, dtype=torch.uint8, device=x.device) if nb else None
-    call("sn_hashgrid_bwd_ws_dev", ptr(x), n, ptr(n_dev), ptr(params), *gargs, ptr(genc),
-         ptr(gparams), ptr(gx), ptr(ws), 0 if ws is None else ws.numel(), stream)
+    call("sn_hashgrid_bwd_ws_dev", ptr(x), n, ptr(n_dev), ptr(params), *gargs,
+         ptr(n_active_dev), ptr(genc), ptr(gparams), ptr(gx), ptr(ws),
+         0 if ws is None else ws.numel(), stream)
 
 
 class _HashBackwardFn(torch.autograd.Function):

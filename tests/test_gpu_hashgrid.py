@@ -124,12 +124,12 @@ def test_bucket_backward_matches_atomics(records, dev):
     base = torch.randn(e.params.shape, generator=g).to(dev) * 1e-3
     s = stream_of(dev)
     g_ref, gx_ref = base.clone(), torch.empty_like(x)
-    call("sn_hashgrid_bwd_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs, ptr(genc),
+    call("sn_hashgrid_bwd_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs, None, ptr(genc),
          ptr(g_ref), ptr(gx_ref), s)
     outs = []
     for _ in range(2):
         g_b, gx_b = base.clone(), torch.empty_like(x)
-        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs,
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs, None,
              ptr(genc), ptr(g_b), ptr(gx_b), ptr(ws), nb, s)
         torch.cuda.synchronize()
         scale = float((g_ref - base).abs().max())
@@ -141,7 +141,7 @@ def test_bucket_backward_matches_atomics(records, dev):
         assert torch.equal(outs[0], outs[1])
     # a workspace too small for one record per bucket is refused
     with pytest.raises(RuntimeError):
-        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs,
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), n_cap, ptr(n_dev), ptr(e.params), *gargs, None,
              ptr(genc), ptr(g_b), None, ptr(ws), 64, s)
 
 
@@ -220,7 +220,7 @@ def test_forward_from_fp16_copy(name, n_active, dev):
     a = torch.empty(n, L * F, device=dev)
     b = torch.empty(n, L * F, device=dev)
     n_dev = torch.tensor([n], dtype=torch.int32, device=dev)
-    call("sn_hashgrid_fwd16_dev", ptr(x), n, ptr(n_dev), ptr(t16), *args, ptr(a), st)
+    call("sn_hashgrid_fwd16_dev", ptr(x), n, ptr(n_dev), ptr(t16), *args, None, ptr(a), st)
     call("sn_hashgrid_fwd", ptr(x), n, ptr(rounded), *args, ptr(b), st)
     assert torch.equal(a, b)
     # Adam refreshes the copy of a sub-range of a larger flat buffer
@@ -231,3 +231,35 @@ def test_forward_from_fp16_copy(name, n_active, dev):
     call("sn_adam_step_copy16", ptr(flat), ptr(grad), ptr(mom1), ptr(mom2), flat.numel(), 1e-2,
          None, 0.9, 0.999, 1e-8, 1, None, ptr(t16), 1001, m, st)
     assert torch.equal(t16, flat[1001:1001 + m].half())
+
+
+@pytest.mark.parametrize("k", [0, 3, 16])
+def test_device_level_count(dev, k):
+    """The active level count read from device memory (n_active_dev, bounded
+    by the host count 16) gives the same forward and backward, bit for bit,
+    as the host count k."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=9)
+    g = torch.Generator().manual_seed(10)
+    x = (torch.rand(5000, 3, generator=g) * 2 - 1).to(dev)
+    genc = torch.randn(5000, 32, generator=g).to(dev)
+    kd = torch.tensor([k], dtype=torch.int32, device=dev)
+    s = stream_of(dev)
+    outs = []
+    for host_k, dev_k in ((k, None), (16, kd)):
+        gargs = (16, 2, 19, 16, 1.38, host_k)
+        enc = torch.empty(5000, 32, device=dev)
+        call("sn_hashgrid_fwd_dev", ptr(x), 5000, None, ptr(e.params), *gargs, ptr(dev_k),
+             ptr(enc), s)
+        nb = _lib.lib().sn_hashgrid_bwd_workspace_bytes(16, 2, 19, 16, 1.38, 65536)
+        ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        gp = torch.zeros_like(e.params)
+        gx = torch.empty_like(x)
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), 5000, None, ptr(e.params), *gargs, ptr(dev_k),
+             ptr(genc), ptr(gp), ptr(gx), ptr(ws), nb, s)
+        torch.cuda.synchronize()
+        outs.append((enc, gp, gx))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
