@@ -468,9 +468,15 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
 /* (8192 table rows per bucket); a second kernel sums each list as 64-bit   */
 /* fixed point scaled by the level's largest value and adds the sums to the */
 /* rows it owns.  Records beyond a list's capacity take fp32 atomics; with  */
-/* none, the result is bit-reproducible.  The capacity follows from          */
-/* workspace_bytes; sn_hashgrid_bwd_workspace_bytes gives the bytes for     */
-/* records_per_bucket (-1: unsupported config).  Same sums as               */
+/* none, the result is bit-reproducible.  The lists share one pool, whose   */
+/* size follows from workspace_bytes (sn_hashgrid_bwd_workspace_bytes gives */
+/* the bytes for records_per_bucket times the bucket count; -1: unsupported */
+/* config).  Each call shares the pool out by the counts the previous call  */
+/* on the same workspace left in it (5/4 of each list's count plus a floor, */
+/* at least the even split, at most 131072 records, scaled to the pool; the */
+/* even split for a workspace no call has used): the                        */
+/* workspace carries this hint between calls, any content is valid input,  */
+/* and calls that share a workspace must not overlap.  Same sums as         */
 /* sn_hashgrid_bwd_dev to fp32 rounding.                                     */
 int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
                                         int32_t log2_hashmap_size, int32_t base_resolution,

@@ -329,13 +329,21 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 // 1. hash_bwd_rec_kernel: a workgroup takes kRecThreads (512) consecutive points and, per
 //    level, turns their corner contributions into records (table row, F
 //    values).  Lanes of a wavefront whose points lie in the same cell (common
-//    on coarse levels: a wavefront holds a few neighbouring samples of one
-//    patch) are summed first with cross-lane (DPP) reductions and leave one
-//    record per corner; the other lanes leave one record per corner each.
+//    on coarse and middle levels: a wavefront holds neighbouring samples of
+//    a few rays of one patch) are summed first with cross-lane (DPP)
+//    reductions and leave one record per corner; the other lanes leave one
+//    record per corner each.
 //    Records go to the list of their bucket (kBucketEntries consecutive rows
 //    of the level): ranks within the workgroup come from integer LDS atomics,
 //    room in each list from one global atomic per (workgroup, level, bucket).
 //    Records beyond a list's capacity are added with fp32 atomics instead.
+//    The lists share one pool; bucket_layout_kernel (before the records)
+//    gives each list 5/4 of the records it took in the previous call on the
+//    same workspace plus a floor (a quarter of the pool split evenly), at
+//    least the even split and at most kMaxList, scaled down to the pool (the
+//    even split when the workspace holds no previous call's layout): with a
+//    pool smaller than kMaxList per bucket, the buckets that take many
+//    records get the room.
 //    The largest |value| of each level is kept (integer atomic max on the
 //    bits of a non-negative float).
 // 2. hash_bwd_bucket_sum_kernel: one workgroup per bucket sums its records in
@@ -349,7 +357,20 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 constexpr int kBucketShift = 13;
 constexpr int kBucketEntries = 1 << kBucketShift;
 constexpr int kMaxBucketsPerLevel = 512;
-constexpr int kMinGroup = 4;  // smallest same-cell lane group summed across lanes
+// Same-cell lane groups of kMinGroup or more lanes are summed across lanes
+// (measured: groups of 2-3 cost more in cross-lane sums than their records
+// cost); the search for groups (one leader per step, lowest pending lane
+// first) stops after kMaxSingles leaders in a row with a smaller group, and
+// is skipped on levels where no lane shares the cell of the lane before it
+// (the fine levels).
+#ifndef SN_MIN_GROUP
+#define SN_MIN_GROUP 4
+#endif
+#ifndef SN_MAX_SINGLES
+#define SN_MAX_SINGLES 6
+#endif
+constexpr int kMinGroup = SN_MIN_GROUP;
+constexpr int kMaxSingles = SN_MAX_SINGLES;
 #ifndef SN_REC_THREADS
 #define SN_REC_THREADS 512
 #endif
@@ -361,12 +382,104 @@ struct BucketDesc {
 };
 
 struct BucketPool {
-  int32_t* cursor;    // [n_buckets] records reserved per bucket (may exceed cap)
+  int32_t* hdr;       // [4] layout stamp: kLayoutMagic, n_buckets, total records
+  int32_t* cursor;    // [n_buckets] records reserved per bucket (may exceed its cap)
   uint32_t* maxbits;  // [kMaxLevels] bits of the largest |record value| per level
-  uint16_t* row;      // [n_buckets * cap] row within the bucket
-  float* val;         // [n_buckets * cap * F]
-  int32_t cap;        // records per bucket
+  int32_t* start;     // [n_buckets] first pool index of each bucket's list
+  int32_t* bcap;      // [n_buckets] list capacity
+  uint16_t* row;      // [total] row within the bucket
+  float* val;         // [total * F]
+  int32_t total;      // records in the pool
 };
+
+constexpr int32_t kLayoutMagic = 0x4c594f54;
+// Largest list: the sum kernel takes a list in one workgroup, whose integer
+// LDS atomics (~5e9 lane ops/s per CU) make a list of n records last at
+// least n * F / 5e9 s; records beyond it take the atomic path.
+constexpr int64_t kMaxList = 131072;
+constexpr int64_t kMaxPoolRecords = 1ll << 30;  // keeps the layout's products in int64
+constexpr int kLayoutThreads = 1024;
+constexpr int kMaxBuckets = kMaxLevels * kMaxBucketsPerLevel;
+
+// One workgroup: list capacities from the previous call's counts (cursor),
+// start offsets (exclusive scan), counts and level maxima back to zero.  Any
+// workspace content gives a valid layout (starts + caps within the pool);
+// only its speed depends on the content.
+__global__ void __launch_bounds__(kLayoutThreads)
+bucket_layout_kernel(BucketPool pool, int32_t nb) {
+  constexpr int kPer = (kMaxBuckets + kLayoutThreads - 1) / kLayoutThreads;
+  constexpr int kWaves = kLayoutThreads / 64;
+  __shared__ long long wsum[kWaves];
+  __shared__ int valid;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t total = pool.total;
+  if (tid == 0)
+    valid = pool.hdr[0] == kLayoutMagic && pool.hdr[1] == nb && pool.hdr[2] == pool.total;
+  __syncthreads();
+  const int per = (nb + kLayoutThreads - 1) / kLayoutThreads;  // <= kPer
+  const int64_t floor_share = max((int64_t)1, total / (4 * (int64_t)nb));
+  const int64_t even = total / nb;
+  int64_t want[kPer];
+  long long s = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int b = tid * per + q;
+    int64_t w = 0;
+    if (q < per && b < nb) {
+      if (valid) {
+        const int64_t prev = min((int64_t)max(pool.cursor[b], 0), total);
+        w = max(even, prev + prev / 4 + floor_share);
+      } else {
+        w = even;
+      }
+      w = min(w, kMaxList);
+    }
+    want[q] = w;
+    s += w;
+  }
+  // block sum
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) wsum[wv] = s;
+  __syncthreads();
+  long long tot = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) tot += wsum[k];
+  __syncthreads();
+  // scale down to the pool (w <= 2.25 total < 2^31.2, total <= 2^30: no overflow)
+  long long mine = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    if (tot > total) want[q] = want[q] * total / tot;
+    mine += want[q];
+  }
+  // exclusive scan of the per-thread sums
+  long long incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  long long run = incl - mine;
+  for (int k = 0; k < wv; ++k) run += wsum[k];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int b = tid * per + q;
+    if (q < per && b < nb) {
+      pool.start[b] = (int32_t)run;
+      pool.bcap[b] = (int32_t)want[q];
+      pool.cursor[b] = 0;
+      run += want[q];
+    }
+  }
+  if (tid < kMaxLevels) pool.maxbits[tid] = 0u;
+  if (tid == 0) {
+    pool.hdr[0] = kLayoutMagic;
+    pool.hdr[1] = nb;
+    pool.hdr[2] = pool.total;
+    pool.hdr[3] = 0;
+  }
+}
 
 // DPP row shift / broadcast (gfx9 encodings: row_shr:n = 0x110 + n,
 // row_bcast:15 = 0x142, row_bcast:31 = 0x143); lanes without a source read 0.
@@ -404,7 +517,8 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
                     float* __restrict__ gx, BucketDesc bd, BucketPool pool) {
   __shared__ int32_t hist[kMaxBucketsPerLevel];   // records per bucket (this level)
   __shared__ int32_t lofs[kMaxBucketsPerLevel];   // their first staging slot
-  __shared__ int32_t rbase[kMaxBucketsPerLevel];  // their first index in the list
+  __shared__ int32_t rroom[kMaxBucketsPerLevel];  // list room left at the workgroup's share
+  __shared__ int32_t rbase[kMaxBucketsPerLevel];  // pool index of the share's first record
   __shared__ StagedRecord stage[kRecThreads * 8];
   __shared__ uint32_t lmax;
   __shared__ int32_t n_staged;
@@ -439,6 +553,14 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         for (int f = 0; f < F; ++f) rv[c][f] = w[c] * gl[f];
       bool has = act;
       uint64_t pend = __ballot(act);
+      {
+        const uint32_t p0 = __shfl_up(cell[0], 1, 64), p1 = __shfl_up(cell[1], 1, 64),
+                       p2 = __shfl_up(cell[2], 1, 64);
+        const bool same_prev = lane > 0 && ((pend >> (lane - 1)) & 1ull) && act &&
+                               cell[0] == p0 && cell[1] == p1 && cell[2] == p2;
+        if (__ballot(same_prev) == 0ull) pend = 0ull;  // uniform
+      }
+      int singles = 0;
       while (pend) {  // uniform
         const int ld = __ffsll((unsigned long long)pend) - 1;
         const uint32_t c0 = __builtin_amdgcn_readlane(cell[0], ld);
@@ -446,15 +568,19 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         const uint32_t c2 = __builtin_amdgcn_readlane(cell[2], ld);
         const bool in = ((pend >> lane) & 1ull) && cell[0] == c0 && cell[1] == c1 && cell[2] == c2;
         const uint64_t m = __ballot(in);
-        if (__popcll(m) < kMinGroup) break;
+        if (__popcll(m) >= kMinGroup) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+          for (int c = 0; c < 8; ++c)
 #pragma unroll
-          for (int f = 0; f < F; ++f) {
-            const float sum = wave_sum(in ? w[c] * gl[f] : 0.f);
-            if (lane == ld) rv[c][f] = sum;
-          }
-        if (in && lane != ld) has = false;
+            for (int f = 0; f < F; ++f) {
+              const float sum = wave_sum(in ? w[c] * gl[f] : 0.f);
+              if (lane == ld) rv[c][f] = sum;
+            }
+          if (in && lane != ld) has = false;
+          singles = 0;
+        } else if (++singles >= kMaxSingles) {
+          break;
+        }
         pend &= ~m;
       }
       // ranks within the workgroup's share of each bucket
@@ -510,7 +636,13 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
             const int32_t c = hist[b];
             lofs[b] = run;
             run += c;
-            if (c) rbase[b] = atomicAdd(&pool.cursor[gb0 + b], c);
+            if (c) {
+              // relative position in the list; the pool index only while in room
+              const int32_t r0 = atomicAdd(&pool.cursor[gb0 + b], c);
+              const int32_t room = pool.bcap[gb0 + b] - r0;
+              rroom[b] = room;
+              rbase[b] = room > 0 ? pool.start[gb0 + b] + r0 : 0;
+            }
             hist[b] = 0;
           }
         }
@@ -525,10 +657,9 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int b = (int)(idx[c] >> kBucketShift);
-          const int32_t r = rbase[b] + rank[c];
           StagedRecord& e = stage[lofs[b] + rank[c]];
           e.key = idx[c];
-          e.at = r < pool.cap ? (gb0 + b) * pool.cap + r : -1;
+          e.at = rank[c] < rroom[b] ? rbase[b] + rank[c] : -1;
 #pragma unroll
           for (int f = 0; f < F; ++f) e.v[f] = rv[c][f];
         }
@@ -565,10 +696,10 @@ hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
   const int gb = blockIdx.x, tid = threadIdx.x;
   int l = 0;
   while (l + 1 < d.n_levels && bd.base[l + 1] <= gb) ++l;
-  const int32_t nrec = min(pool.cursor[gb], pool.cap);
+  const int32_t nrec = min(pool.cursor[gb], pool.bcap[gb]);
   const float vmax = __uint_as_float(pool.maxbits[l]);
   if (nrec <= 0 || vmax == 0.f) return;  // uniform per workgroup
-  const int64_t at0 = (int64_t)gb * pool.cap;
+  const int64_t at0 = pool.start[gb];
   const int64_t e0 = (int64_t)(gb - bd.base[l]) * kBucketEntries;
   const int64_t rows = min((int64_t)kBucketEntries, (int64_t)d.hsize[l] - e0);
   float* tg = gparams + ((int64_t)d.offset[l] + e0) * F;
@@ -621,13 +752,14 @@ static int64_t make_bucket_desc(const HashGridDesc& d, BucketDesc* bd) {
   return tot;
 }
 
-// workspace: cursor[n_buckets] | maxbits[kMaxLevels] | row[n_buckets*cap] | val[n_buckets*cap*F]
+// workspace: hdr[4] | cursor[nb] | maxbits[kMaxLevels] | start[nb] | bcap[nb] |
+//            row[total] | val[total*F]
 static size_t bucket_head_bytes(int64_t n_buckets) {
-  return align_up(sizeof(int32_t) * (n_buckets + kMaxLevels), 256);
+  return align_up(sizeof(int32_t) * (4 + 3 * n_buckets + kMaxLevels), 256);
 }
-static size_t bucket_ws_bytes(int64_t n_buckets, int F, int64_t cap) {
-  return bucket_head_bytes(n_buckets) + align_up(sizeof(uint16_t) * n_buckets * cap, 256) +
-         align_up(sizeof(float) * F * n_buckets * cap, 256);
+static size_t bucket_ws_bytes(int64_t n_buckets, int F, int64_t total) {
+  return bucket_head_bytes(n_buckets) + align_up(sizeof(uint16_t) * total, 256) +
+         align_up(sizeof(float) * F * total, 256);
 }
 
 // Cotangent g_gx of grad_x = sum_{l,f,c} genc[l,f] * scale_l * dw[c] . theta[c,f]:
@@ -819,7 +951,9 @@ int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
     return -1;
   const int64_t nb = make_bucket_desc(d, &bd);
   if (nb < 0) return -1;
-  return (int64_t)bucket_ws_bytes(nb, n_features, records_per_bucket);
+  const int64_t want = (int64_t)records_per_bucket * nb;
+  const int64_t total = want < kMaxPoolRecords ? want : kMaxPoolRecords;
+  return (int64_t)bucket_ws_bytes(nb, n_features, total);
 }
 
 int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
@@ -841,8 +975,6 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
   SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
   hipStream_t s = as_stream(stream);
   const unsigned grid = grid_for(n_cap, 256, kHashBlocks);
-  BucketDesc bd{};
-  BucketPool pool{nullptr, nullptr, nullptr, 0};
   if (g_aggregate == 2 && grad_params && n_features <= 2) {
     // bucket records when a workspace is given (and the 32-bit record
     // counters cannot overflow), else atomics from the workgroup merge
@@ -851,25 +983,29 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
     const bool buckets = workspace && nb > 0 && n_cap < (1ll << 28) - 4096 && d.n_active > 0;
     if (buckets) {
       const size_t head = bucket_head_bytes(nb);
-      const size_t per = (sizeof(uint16_t) + sizeof(float) * n_features) * (size_t)nb;
-      int64_t cap =
+      const size_t per = sizeof(uint16_t) + sizeof(float) * n_features;
+      int64_t total =
           workspace_bytes > head + 512 ? (int64_t)((workspace_bytes - head - 512) / per) : 0;
-      if (cap > (1 << 24)) cap = 1 << 24;
-      while (cap > 0 && bucket_ws_bytes(nb, n_features, cap) > workspace_bytes) --cap;
-      if (cap <= 0) {
-        set_error("sn_hashgrid_bwd: workspace of %zu bytes holds no bucket records",
+      if (total > kMaxPoolRecords) total = kMaxPoolRecords;
+      while (total > 0 && bucket_ws_bytes(nb, n_features, total) > workspace_bytes) --total;
+      if (total < nb) {
+        set_error("sn_hashgrid_bwd: workspace of %zu bytes holds less than one record per bucket",
                   workspace_bytes);
         return SN_ESPACE;
       }
       char* w = static_cast<char*>(workspace);
       BucketPool pool;
-      pool.cursor = reinterpret_cast<int32_t*>(w);
-      pool.maxbits = reinterpret_cast<uint32_t*>(w) + nb;
+      pool.hdr = reinterpret_cast<int32_t*>(w);
+      pool.cursor = pool.hdr + 4;
+      pool.maxbits = reinterpret_cast<uint32_t*>(pool.cursor + nb);
+      pool.start = pool.cursor + nb + kMaxLevels;
+      pool.bcap = pool.start + nb;
       pool.row = reinterpret_cast<uint16_t*>(w + head);
-      pool.val = reinterpret_cast<float*>(w + head + align_up(sizeof(uint16_t) * nb * cap, 256));
-      pool.cap = (int32_t)cap;
+      pool.val = reinterpret_cast<float*>(w + head + align_up(sizeof(uint16_t) * total, 256));
+      pool.total = (int32_t)total;
       const int32_t nb_active = bd.base[d.n_active];
-      rc = fill_i32(pool.cursor, nb + kMaxLevels, 0, s);  // cursors and level maxima
+      bucket_layout_kernel<<<1, kLayoutThreads, 0, s>>>(pool, (int32_t)nb);
+      rc = check_launch("sn_hashgrid_bwd");
       if (rc) return rc;
       if (n_features == 1)
         hash_bwd_rec_kernel<1><<<grid_for(n_cap, kRecThreads, kHashBlocks), kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,

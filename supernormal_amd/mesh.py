@@ -21,29 +21,32 @@ import torch
 
 
 @torch.no_grad()
-def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=1 << 16):
+def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=1 << 18,
+               check_every=16):
     """First + to - crossing of sdf_fn along each ray in [near, far]; returns
-    (points [M,3], hit mask [N])."""
+    (points [M,3], hit mask [N]).  Every step evaluates all rays of the chunk
+    (finished rays keep their state), so a step is a fixed set of launches;
+    whether all rays are finished is read back every `check_every` steps
+    (the extra steps change nothing)."""
     dev = rays_o.device
     N = rays_o.shape[0]
     pts = torch.zeros(N, 3, device=dev)
     hit = torch.zeros(N, dtype=torch.bool, device=dev)
     for s in range(0, N, chunk):
         o, d = rays_o[s:s + chunk], rays_d[s:s + chunk]
-        t_lo = near[s:s + chunk].clone()
+        t = near[s:s + chunk].clone()
         t_hi_lim = far[s:s + chunk]
-        f_lo = sdf_fn(o + d * t_lo[:, None])
+        f_lo = sdf_fn(o + d * t[:, None])
         done = torch.zeros(o.shape[0], dtype=torch.bool, device=dev)
         found = torch.zeros_like(done)
-        t_hit_lo = torch.zeros_like(t_lo)
-        t_hit_hi = torch.zeros_like(t_lo)
-        t = t_lo.clone()
-        while not bool(done.all()):
+        t_hit_lo = torch.zeros_like(t)
+        t_hit_hi = torch.zeros_like(t)
+        one = torch.ones_like(f_lo)
+        k = 0
+        while k % check_every != 0 or not bool(done.all()):
             act = ~done
             t_new = torch.where(act, t + step, t)
-            f_new = torch.full_like(f_lo, 1.0)
-            idx = act.nonzero()[:, 0]
-            f_new[idx] = sdf_fn(o[idx] + d[idx] * t_new[idx, None])
+            f_new = torch.where(act, sdf_fn(o + d * t_new[:, None]), one)
             cross = act & (f_lo > 0) & (f_new <= 0)
             t_hit_lo = torch.where(cross, t, t_hit_lo)
             t_hit_hi = torch.where(cross, t_new, t_hit_hi)
@@ -51,6 +54,7 @@ def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=
             done |= cross | (t_new >= t_hi_lim)
             t = t_new
             f_lo = torch.where(act, f_new, f_lo)
+            k += 1
         a, b = t_hit_lo, t_hit_hi
         for _ in range(n_bisect):
             m = 0.5 * (a + b)
@@ -65,9 +69,10 @@ def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=
 
 @torch.no_grad()
 def visible_surface_points(sdf_fn, dataset, rays_per_view=20000, seed=0, step=2e-3):
-    """First hits of the learned SDF along masked training-pixel rays."""
+    """First hits of the learned SDF along masked training-pixel rays (the
+    rays of all views marched together)."""
     g = torch.Generator(device="cpu").manual_seed(seed)
-    out = []
+    rays = []
     for i in dataset.train_images:
         m = dataset.masks[i].reshape(-1).nonzero()[:, 0].cpu()
         if m.numel() == 0:
@@ -79,10 +84,14 @@ def visible_surface_points(sdf_fn, dataset, rays_per_view=20000, seed=0, step=2e
         d = p / p.norm(dim=-1, keepdim=True)
         d = d @ dataset.pose_all[i, :3, :3].T
         o = dataset.pose_all[i, :3, 3].expand(d.shape).contiguous()
-        near, far = dataset.near_far_from_sphere(o, d)
-        pts, _ = first_hits(sdf_fn, o, d, near, far, step=step)
-        out.append(pts)
-    return torch.cat(out).cpu().numpy()
+        rays.append((o, d))
+    if not rays:
+        return np.zeros((0, 3), np.float32)
+    o = torch.cat([r[0] for r in rays])
+    d = torch.cat([r[1] for r in rays])
+    near, far = dataset.near_far_from_sphere(o, d)
+    pts, _ = first_hits(sdf_fn, o, d, near, far, step=step)
+    return pts.cpu().numpy()
 
 
 @torch.no_grad()

@@ -77,7 +77,8 @@ def hash_entries(n_levels, log2_T, base, scale):
 
 def job_bytes(spec: JobSpec, max_samples_per_ray=1024, n_levels=16, F=2, patch=9) -> int:
     """Resident device bytes of a job: parameters with gradient and Adam
-    moments, the step buffers at capacity, the dataset and the occupancy grid."""
+    moments, the step buffers at capacity, the hash-grid backward's record
+    pool (tcnn.bucket_workspace_bytes), the dataset and the occupancy grid."""
     E = n_levels * F
     params = hash_entries(n_levels, spec.log2_hashmap_size, 16, 1.38) * F + 8000
     c0 = spec.batch_size * max_samples_per_ray
@@ -85,7 +86,9 @@ def job_bytes(spec: JobSpec, max_samples_per_ray=1024, n_levels=16, F=2, patch=9
     step = c1 * (3 + 2 * E + 2) * 4 + c0 * (2 * 3 + E + 1) * 4 * 2 + c0 * 40
     data = spec.n_views * spec.resolution ** 2 * (12 + 4 + 36)
     occ = 128 ** 3 * 5
-    return int(params * 16 + step + data + occ)
+    rec = 2 + 4 * F
+    hash_ws = min(8 * n_levels * c1, 131072 * 747, (4 << 30) // rec) * rec
+    return int(params * 16 + step + hash_ws + data + occ)
 
 
 def job_data(spec: JobSpec, device):
@@ -205,6 +208,8 @@ class Server:
         used = 0
         t0 = self.clock()
         peak_running = 0
+        # host seconds per phase of the loop (job set-up, issue, evaluation, finish)
+        phase = dict(admit=0.0, issue=0.0, evaluate=0.0, finish=0.0)
         while queue or running:
             now = self.clock() - t0
             # admit, in arrival order, while slots and memory allow
@@ -213,6 +218,7 @@ class Server:
                 spec = queue.pop(0)
                 t_admit = self.clock() - t0
                 job = self.make_job(spec)
+                phase["admit"] += self.clock() - t0 - t_admit
                 job.t_arrival, job.t_admit = spec.arrival_s, t_admit
                 job.bytes = job_bytes(spec)
                 used += job.bytes
@@ -222,15 +228,21 @@ class Server:
                 if queue:
                     self.sleep(max(0.0, queue[0].arrival_s - (self.clock() - t0)))
                 continue
+            tp = self.clock()
             for job in running:
                 job.advance(self.chunk)
+            phase["issue"] += self.clock() - tp
             done = []
             for job in running:
                 if job.due_for_eval():
+                    tp = self.clock()
                     job.evaluate()
+                    phase["evaluate"] += self.clock() - tp
                     if job.finished():
+                        tp = self.clock()
                         job.finish()
                         job.t_done = self.clock() - t0
+                        phase["finish"] += job.t_done + t0 - tp
                         done.append(job)
             for job in done:
                 running.remove(job)
@@ -251,7 +263,8 @@ class Server:
         return dict(records=records, p50_s=float(np.percentile(ttm, 50)),
                     p99_s=float(np.percentile(ttm, 99)), makespan_s=t_end,
                     samples_per_s=float(sum(r["samples"] for r in records)) / max(t_end, 1e-9),
-                    peak_running=peak_running)
+                    peak_running=peak_running,
+                    host_s={k: round(v, 3) for k, v in phase.items()})
 
 
 def run_stream(specs, device="cuda", one_at_a_time=False, **kw):

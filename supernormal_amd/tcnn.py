@@ -51,25 +51,28 @@ def _gargs(g, n_active):
             g["per_level_scale"], int(n_active))
 
 
-_REC_MIN, _REC_MAX, _WS_MAX = 4096, 131072, 2 << 30
+_REC_MIN, _REC_MAX, _WS_MAX = 64, 131072, 4 << 30
 
 
 def bucket_workspace_bytes(gargs, n_points):
     """Bytes of the bucket workspace of the hash-grid backward over up to
-    n_points points (0 when the config has no bucket path).  Records per
-    bucket: twice the mean count if every point had 8 distinct corners per
-    level spread over the finest level's buckets, within [4096, 131072] and at
-    most ~2 GiB in total; records beyond it take the atomic path."""
+    n_points points (0 when the config has no bucket path).  The pool holds a
+    record for every corner of every level of every point (8 per point and
+    level), at most ~4 GiB; the kernels share it out over the buckets by the
+    counts of the previous call on the same workspace (at most 131072 per
+    bucket, the kernels' list limit), and records beyond a bucket's share
+    take the atomic path."""
     n_levels, F, log2T, base, pls = gargs[:5]
     lib = _lib.lib()
     one = lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, 1)
     if one <= 0:
         return 0
-    n_buckets = max(1, one // (2 + 4 * F))
-    per_level = max(1, ((1 << log2T) + 8191) // 8192)
-    cap = 16 * int(n_points) // per_level
-    cap = min(_REC_MAX, max(_REC_MIN, cap), max(1, _WS_MAX // (n_buckets * (2 + 4 * F))))
-    return int(lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, int(cap)))
+    per = 2 + 4 * F
+    more = lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, 1025)
+    n_buckets = max(1, round((more - one) / (1024 * per)))
+    total = min(8 * n_levels * max(int(n_points), 1), _WS_MAX // per)
+    rpb = min(_REC_MAX, max(_REC_MIN, -(-total // n_buckets)))
+    return int(lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, int(rpb)))
 
 
 def hashgrid_bwd(x, n, n_dev, params, gargs, genc, gparams, gx, stream, ws=None,

@@ -145,6 +145,50 @@ def test_bucket_backward_matches_atomics(records, dev):
              ptr(genc), ptr(g_b), None, ptr(ws), 64, s)
 
 
+def test_bucket_pool_follows_previous_counts(dev):
+    """The record pool is shared out by the counts of the previous call on the
+    same workspace.  12000 uniform points leave ~96k records in the single
+    bucket of level 0: the even split of a fresh workspace (16384 per bucket)
+    overflows there (atomics), the second call's lists hold every record, and
+    from then on calls repeat bit for bit; all agree with the atomic path."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=11)
+    g = torch.Generator().manual_seed(12)
+    n = 12000
+    x = (torch.rand(n, 3, generator=g) * 2 - 1).to(dev)
+    genc = torch.randn(n, 32, generator=g).to(dev)
+    gargs = (16, 2, 19, 16, 1.38, 16)
+    lib = _lib.lib()
+    b1 = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], 1)
+    b2 = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], 1025)
+    nbk = round((b2 - b1) / (1024 * 10))
+    assert nbk == 747
+    nbytes = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], 16384)
+    ws = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+    head = ws[:4 * (4 + 3 * nbk + 32)].view(torch.int32)
+    cursor = lambda: head[4:4 + nbk].cpu()
+    bcap = lambda: head[4 + 2 * nbk + 32:4 + 3 * nbk + 32].cpu()
+    s = stream_of(dev)
+    g_ref = torch.zeros_like(e.params)
+    call("sn_hashgrid_bwd_dev", ptr(x), n, None, ptr(e.params), *gargs, None, ptr(genc),
+         ptr(g_ref), None, s)
+    scale = float(g_ref.abs().max())
+    outs, over = [], []
+    for _ in range(3):
+        gp = torch.zeros_like(e.params)
+        call("sn_hashgrid_bwd_ws_dev", ptr(x), n, None, ptr(e.params), *gargs, None, ptr(genc),
+             ptr(gp), None, ptr(ws), nbytes, s)
+        torch.cuda.synchronize()
+        over.append(int((cursor() - bcap()).clamp_min(0).sum()))
+        torch.testing.assert_close(gp, g_ref, rtol=1e-4, atol=1e-5 * scale)
+        outs.append(gp)
+    assert over[0] > 30000 and over[1] == 0 and over[2] == 0
+    assert int(cursor()[0]) > 80000  # the level-0 list
+    assert torch.equal(outs[1], outs[2])
+
+
 def test_double_backward_vs_fp64(dev):
     """L = sum(v * dL1/dx) with dL1/dx = autograd.grad(sum(g * enc), x, create_graph=True):
     d L / d params, d L / d g (through enc's upstream gradient) and d L / d x."""
@@ -254,7 +298,9 @@ def test_device_level_count(dev, k):
         call("sn_hashgrid_fwd_dev", ptr(x), 5000, None, ptr(e.params), *gargs, ptr(dev_k),
              ptr(enc), s)
         nb = _lib.lib().sn_hashgrid_bwd_workspace_bytes(16, 2, 19, 16, 1.38, 65536)
-        ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        # a fresh (zeroed) workspace each: a workspace carries the previous
+        # call's list sizes, which could make another input's lists overflow
+        ws = torch.zeros(nb, dtype=torch.uint8, device=dev)
         gp = torch.zeros_like(e.params)
         gx = torch.empty_like(x)
         call("sn_hashgrid_bwd_ws_dev", ptr(x), 5000, None, ptr(e.params), *gargs, ptr(dev_k),
