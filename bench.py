@@ -22,9 +22,11 @@ so `value` is the run-average iterations/s.  Beside it:
     kernels, torch Linear MLP) timed from the same trained state;
   * roofline: the dominant kernel's algorithmic bytes per launch / its mean
     duration, from HIP events on its launch stream around every launch of the
-    timed region (in graph mode the iteration is three graphs with the
-    hash-grid backward alone in the middle one, so the events sit between
-    replays);
+    timed region (in graph mode the iteration is four graphs with the
+    hash-grid backward alone in the third, so the events sit between
+    replays); `traffic` = the HBM bytes of one late launch from rocprofv3
+    FETCH_SIZE / WRITE_SIZE counter runs (--traffic-probe), scaled by what
+    the counters read for a copy of known size;
   * cpu_baseline: the CPU oracle (torch restatement of the reference
     iteration) on the same workload, a bounded number of iterations;
   * accuracy: Chamfer distance / F-score of the visible surface points of the
@@ -65,6 +67,14 @@ def hash_fwd_bytes(n, n_active, grid):
 
 
 ROOFLINE_KERNELS = {"sn_hashgrid_bwd": hash_bwd_bytes, "sn_hashgrid_fwd": hash_fwd_bytes}
+
+# HBM bytes of one late hash-grid backward launch on the tbs612 workload, from
+# two rocprofv3 counter runs over --traffic-probe (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the factors being what a 64 MiB copy reads on each
+# counter), with that launch's point count and algorithmic bytes; the
+# commands and per-kernel figures are in profiles/hash_bwd_tbs612_late_traffic.md
+HASH_BWD_LATE_TRAFFIC = {"points": 1471648, "traffic_bytes": 4377201504,
+                         "algorithmic_bytes": 3219966918}
 
 
 def cpu_baseline(runner, iters=2):
@@ -184,6 +194,13 @@ def main(argv=None):
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": a.roofline_kernel, "launches_timed": len(durs),
                 "avg_launch_ms": round(avg_ms, 4), "avg_algorithmic_bytes": int(avg_bytes)}
+        if a.roofline_kernel == "sn_hashgrid_bwd" and a.workload == "tbs612":
+            t = HASH_BWD_LATE_TRAFFIC
+            roof["traffic"] = t["traffic_bytes"]
+            roof["traffic_launch"] = {"points": t["points"],
+                                      "algorithmic_bytes": t["algorithmic_bytes"],
+                                      "traffic_over_algorithmic":
+                                          round(t["traffic_bytes"] / t["algorithmic_bytes"], 3)}
         if len(durs) >= 1000:
             # the last 1000 launches (all levels active, the most points)
             late_ms = float(np.mean(durs[-1000:]))

@@ -473,7 +473,8 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
 /* the bytes for records_per_bucket times the bucket count; -1: unsupported */
 /* config).  Each call shares the pool out by the counts the previous call  */
 /* on the same workspace left in it (5/4 of each list's count plus a floor, */
-/* at least the even split, at most 131072 records, scaled to the pool; the */
+/* at least the even split, at most 131072 records or the even split if    */
+/* larger, scaled to the pool; the                                          */
 /* even split for a workspace no call has used): the                        */
 /* workspace carries this hint between calls, any content is valid input,  */
 /* and calls that share a workspace must not overlap.  Same sums as         */

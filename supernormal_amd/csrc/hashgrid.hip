@@ -56,6 +56,12 @@ __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restric
                                                float* __restrict__ enc) {
   const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
   float* out = enc + i * (int64_t)(d.n_levels * F);
+  // F = 2 pairs: the outputs of 4 levels are written together (two 16-byte
+  // stores) when the rows allow it, instead of 8 bytes per level into lines
+  // the L2 does not keep between levels
+  const bool vec_out = F == 2 && kPair && (d.n_levels & 3) == 0 &&
+                       (reinterpret_cast<uintptr_t>(enc) & 15) == 0;
+  float o0 = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f, o4 = 0.f, o5 = 0.f, o6 = 0.f, o7 = 0.f;
   for (int l = 0; l < d.n_levels; ++l) {
     float acc[F];
 #pragma unroll
@@ -87,8 +93,28 @@ __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restric
       }
     }
     if constexpr (F == 2 && kPair) {
-      typedef float f2_t __attribute__((ext_vector_type(2)));
-      *reinterpret_cast<f2_t*>(out + l * F) = f2_t{acc[0], acc[1]};
+      if (vec_out) {
+        const int k = l & 3;
+        if (k == 0) {
+          o0 = acc[0];
+          o1 = acc[1];
+        } else if (k == 1) {
+          o2 = acc[0];
+          o3 = acc[1];
+        } else if (k == 2) {
+          o4 = acc[0];
+          o5 = acc[1];
+        } else {
+          o6 = acc[0];
+          o7 = acc[1];
+          float4* o4p = reinterpret_cast<float4*>(out + (l - 3) * F);
+          o4p[0] = make_float4(o0, o1, o2, o3);
+          o4p[1] = make_float4(o4, o5, o6, o7);
+        }
+      } else {
+        typedef float f2_t __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<f2_t*>(out + l * F) = f2_t{acc[0], acc[1]};
+      }
     } else {
 #pragma unroll
       for (int f = 0; f < F; ++f) out[l * F + f] = acc[f];
@@ -340,7 +366,8 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    The lists share one pool; bucket_layout_kernel (before the records)
 //    gives each list 5/4 of the records it took in the previous call on the
 //    same workspace plus a floor (a quarter of the pool split evenly), at
-//    least the even split and at most kMaxList, scaled down to the pool (the
+//    least the even split and at most kMaxList (or the even split if
+//    larger), scaled down to the pool (the
 //    even split when the workspace holds no previous call's layout): with a
 //    pool smaller than kMaxList per bucket, the buckets that take many
 //    records get the room.
@@ -358,13 +385,13 @@ constexpr int kBucketShift = 13;
 constexpr int kBucketEntries = 1 << kBucketShift;
 constexpr int kMaxBucketsPerLevel = 512;
 // Same-cell lane groups of kMinGroup or more lanes are summed across lanes
-// (measured: groups of 2-3 cost more in cross-lane sums than their records
-// cost); the search for groups (one leader per step, lowest pending lane
+// (measured on a full training run: a minimum of 2 or 3 costs more in
+// cross-lane sums than it saves in records, 4 to 6 measure alike); the search for groups (one leader per step, lowest pending lane
 // first) stops after kMaxSingles leaders in a row with a smaller group, and
 // is skipped on levels where no lane shares the cell of the lane before it
 // (the fine levels).
 #ifndef SN_MIN_GROUP
-#define SN_MIN_GROUP 4
+#define SN_MIN_GROUP 6
 #endif
 #ifndef SN_MAX_SINGLES
 #define SN_MAX_SINGLES 6
@@ -393,9 +420,10 @@ struct BucketPool {
 };
 
 constexpr int32_t kLayoutMagic = 0x4c594f54;
-// Largest list: the sum kernel takes a list in one workgroup, whose integer
-// LDS atomics (~5e9 lane ops/s per CU) make a list of n records last at
-// least n * F / 5e9 s; records beyond it take the atomic path.
+// Largest list unless the pool's even split is larger: the sum kernel takes
+// a list in one workgroup, whose integer LDS atomics (~5e9 lane ops/s per CU)
+// make a list of n records last at least n * F / 5e9 s; records beyond it
+// take the atomic path.
 constexpr int64_t kMaxList = 131072;
 constexpr int64_t kMaxPoolRecords = 1ll << 30;  // keeps the layout's products in int64
 constexpr int kLayoutThreads = 1024;
@@ -432,7 +460,7 @@ bucket_layout_kernel(BucketPool pool, int32_t nb) {
       } else {
         w = even;
       }
-      w = min(w, kMaxList);
+      w = min(w, max(even, kMaxList));
     }
     want[q] = w;
     s += w;
@@ -538,10 +566,30 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
     }
     const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
     float dx[3] = {0.f, 0.f, 0.f};
+    // dL/denc of 4 levels per load when the rows allow 16-byte loads (F = 2):
+    // a point's row is read in whole 32-byte pieces instead of 8 bytes per
+    // level, which the L2 (4 MB per XCD against ~4 MB of rows in flight per
+    // XCD) does not keep between levels
+    const bool vec_g = F == 2 && (d.n_levels & 3) == 0 &&
+                       (reinterpret_cast<uintptr_t>(genc) & 15) == 0;
+    float4 gq0 = make_float4(0.f, 0.f, 0.f, 0.f), gq1 = gq0;
     for (int l = 0, na_ = active_levels(d); l < na_; ++l) {
       float gl[F];
+      if (vec_g) {
+        if ((l & 3) == 0 && act) {
+          const float4* g4 = reinterpret_cast<const float4*>(g + l * F);
+          gq0 = g4[0];
+          gq1 = g4[1];
+        }
+        const int k = l & 3;
+        const float a0 = k == 0 ? gq0.x : k == 1 ? gq0.z : k == 2 ? gq1.x : gq1.z;
+        const float a1 = k == 0 ? gq0.y : k == 1 ? gq0.w : k == 2 ? gq1.y : gq1.w;
+        gl[0] = a0;
+        gl[F - 1] = a1;
+      } else {
 #pragma unroll
-      for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
+        for (int f = 0; f < F; ++f) gl[f] = act ? g[l * F + f] : 0.f;
+      }
       uint32_t idx[8], cell[3];
       float w[8], fr[3];
       hg_level(p, d, l, idx, w, fr, cell);

@@ -87,7 +87,7 @@ def job_bytes(spec: JobSpec, max_samples_per_ray=1024, n_levels=16, F=2, patch=9
     data = spec.n_views * spec.resolution ** 2 * (12 + 4 + 36)
     occ = 128 ** 3 * 5
     rec = 2 + 4 * F
-    hash_ws = min(8 * n_levels * c1, 131072 * 747, (4 << 30) // rec) * rec
+    hash_ws = min(8 * n_levels * c1, 196608 * 747, (4 << 30) // rec) * rec
     return int(params * 16 + step + hash_ws + data + occ)
 
 
