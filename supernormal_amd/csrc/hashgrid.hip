@@ -771,7 +771,28 @@ hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
   const double scale = ldexp(1.0, K), inv = ldexp(1.0, -K);
   for (int j = tid; j < kBucketEntries * F; j += 1024) acc[j] = 0ull;
   __syncthreads();
-  for (int r = tid; r < nrec; r += 1024) {
+  // 4 records per lane in flight: their loads are issued before the first
+  // LDS atomic (one workgroup per CU, so the loads' latency is hidden here)
+  constexpr int kU = 4;
+  int r = tid;
+  for (; r + (kU - 1) * 1024 < nrec; r += kU * 1024) {
+    int row[kU];
+    float v[kU][F];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      row[u] = pool.row[at0 + r + u * 1024];
+#pragma unroll
+      for (int f = 0; f < F; ++f) v[u][f] = pool.val[(at0 + r + u * 1024) * F + f];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const long long q = llrint((double)v[u][f] * scale);
+        atomicAdd(&acc[row[u] * F + f], (unsigned long long)q);
+      }
+  }
+  for (; r < nrec; r += 1024) {
     const int row = pool.row[at0 + r];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
