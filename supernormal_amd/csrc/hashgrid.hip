@@ -771,9 +771,12 @@ hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
   const double scale = ldexp(1.0, K), inv = ldexp(1.0, -K);
   for (int j = tid; j < kBucketEntries * F; j += 1024) acc[j] = 0ull;
   __syncthreads();
-  // 4 records per lane in flight: their loads are issued before the first
+  // kU records per lane in flight: their loads are issued before the first
   // LDS atomic (one workgroup per CU, so the loads' latency is hidden here)
-  constexpr int kU = 4;
+#ifndef SN_SUM_UNROLL
+#define SN_SUM_UNROLL 4
+#endif
+  constexpr int kU = SN_SUM_UNROLL;
   int r = tid;
   for (; r + (kU - 1) * 1024 < nrec; r += kU * 1024) {
     int row[kU];
