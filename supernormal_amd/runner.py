@@ -156,7 +156,7 @@ class Runner:
                                        if graph else None)
             self.iter_step += 1
             self.update_learning_rate()
-            return losses[0] + losses[1] + losses[2]
+            return losses[:3].sum()
         o, d, plane_n, v_inv, n_true, mask = self.dataset.gen_random_patches(
             self.batch_size, patch_H=self.patch_size, patch_W=self.patch_size)
         pc = self.patch_size // 2

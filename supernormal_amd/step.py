@@ -100,6 +100,10 @@ class SyncFreeStep:
         self.E = cfg.E
         self._alloc()
         self.adam_t.fill_(int(flat.step_count))
+        # host copies of what adam_t / levels_dev hold, so that unchanged values
+        # are not written again (a launch each, ahead of every iteration)
+        self._adam_t_host = int(flat.step_count)
+        self._levels_host = None
         self.graph = None
         # the next iteration's march when it ran ahead: (event, step size)
         self._pending = None
@@ -423,9 +427,16 @@ class SyncFreeStep:
         step.  The caller must not change the occupancy grid before the next
         step (the march has read it) and passes the same step size to it."""
         self.lr.fill_(float(lr))
-        self.levels_dev.fill_(int(self.net.active_levels()))
-        # the optimizer's step count is the host's (other drivers may have stepped it)
-        self.adam_t.fill_(int(self.flat.step_count))
+        levels = int(self.net.active_levels())
+        if levels != self._levels_host:
+            self.levels_dev.fill_(levels)
+            self._levels_host = levels
+        # the optimizer's step count is the host's (other drivers may have stepped
+        # it); the iteration's Adam step increments the device copy itself
+        t = int(self.flat.step_count)
+        if t != self._adam_t_host:
+            self.adam_t.fill_(t)
+        self._adam_t_host = None  # unknown until the iteration is submitted
         timing = self.time_hash_bwd
         if not use_graph or inputs is not None:
             if self._pending is not None:
@@ -438,6 +449,7 @@ class SyncFreeStep:
                 e[1].record()
             self._part_adam()
             self.flat.step_count += 1
+            self._adam_t_host = t + 1
             return self.losses
         if self._pending is None:
             self.step_size.fill_(float(step_size))
@@ -471,6 +483,7 @@ class SyncFreeStep:
         if done is not None:
             self._march_ahead(done, float(next_step_size))
         self.flat.step_count += 1
+        self._adam_t_host = t + 1
         return self.losses
 
     def _march_ahead(self, done, step_size):
