@@ -73,8 +73,8 @@ ROOFLINE_KERNELS = {"sn_hashgrid_bwd": hash_bwd_bytes, "sn_hashgrid_fwd": hash_f
 # WRITE_SIZE x 1.0, the factors being what a 64 MiB copy reads on each
 # counter), with that launch's point count and algorithmic bytes; the
 # commands and per-kernel figures are in profiles/hash_bwd_tbs612_late_traffic.md
-HASH_BWD_LATE_TRAFFIC = {"points": 1472926, "traffic_bytes": 2735576773,
-                         "algorithmic_bytes": 3222763182}
+HASH_BWD_LATE_TRAFFIC = {"points": 1473021, "traffic_bytes": 2740882158,
+                         "algorithmic_bytes": 3222969948}
 
 
 def cpu_baseline(runner, iters=2):
