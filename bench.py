@@ -121,6 +121,8 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
+    ap.add_argument("--seed", type=int, default=0,
+                    help="seed of the initial weights and of the synthetic views' noise")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the timed run: 5 hash-grid backward calls on the last "
                          "iteration's points and 3 copies of 64 MiB (calibration), for "
@@ -140,7 +142,7 @@ def main(argv=None):
     conf = synthetic_conf(a.workload, gradient_method=a.method,
                           end_iter=max(a.steps + a.warmup, 1), mlp_precision=a.precision,
                           step_mode=mode, max_samples_per_ray=a.max_samples)
-    runner = Runner(conf, fused=True, device=dev)
+    runner = Runner(conf, fused=True, device=dev, seed=a.seed)
     sf = runner.sf_step
     grid = runner.sdf_network.encoding.grid
     runner.update_learning_rate()
