@@ -192,9 +192,12 @@ int sn_ray_march_padded_bricks(const float* rays_o, const float* rays_d, const f
 /* = grid[x][y][z] != 0; resolutions must be multiples of 4.                 */
 int sn_occ_pack_bricks(const uint8_t* binary, int32_t rx, int32_t ry, int32_t rz,
                        uint64_t* bricks, void* stream);
+/* n_capped (device int32, nullable): incremented once per ray whose count  */
+/* reached max_per_ray (the cap may have truncated it; the reference marches */
+/* unbounded), accumulated over calls -- the caller reads and clears it.     */
 int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_ray,
                     const float* t_starts_pad, const float* t_ends_pad, int64_t* ray_indices,
-                    float* t_starts, float* t_ends, void* stream);
+                    float* t_starts, float* t_ends, int32_t* n_capped, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Patch accumulation: replaces accumulate_along_rays_patch_based            */
@@ -382,11 +385,19 @@ int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_a
                  double eps, int32_t step, const int32_t* step_dev, void* stream);
 /* The same step, also refreshing an fp16 copy (copy16, nullable) of params   */
 /* [copy_offset, copy_offset + copy_n): the gather copy of the hash table     */
-/* (tcnn keeps its parameters' fp16 copy the same way).                       */
+/* (tcnn keeps its parameters' fp16 copy the same way).  run_dev (device     */
+/* int32, nullable): when *run_dev <= 0 nothing is updated -- the iteration   */
+/* had no sample (exp_runner.py:182 never reaches optimizer.step then).      */
 int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                         int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                         double eps, int32_t step, const int32_t* step_dev, void* copy16,
-                        int64_t copy_offset, int64_t copy_n, void* stream);
+                        int64_t copy_offset, int64_t copy_n, const int32_t* run_dev,
+                        void* stream);
+/* Device step count of the optimiser (torch.optim.Adam's state['step'],     */
+/* exp_runner.py:207): ++*step_dev when *run_dev > 0, else ++*skipped_dev    */
+/* (nullable) -- the count of iterations skipped for want of samples.        */
+int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_dev,
+                    void* stream);
 /* dst[i] = (fp16) src[i] (initialises such a copy). */
 int sn_to_half(const float* src, void* dst, int64_t n, void* stream);
 

@@ -63,45 +63,42 @@ def patch_weights(alpha, packed):
     return w
 
 
-def render_losses(rays_o_all, rays_d_all, plane_n, v_inv, n_true, mask, packed, ri, t0, t1,
-                  sdf_s, sdf_e, inv_s, grad_s=None, normal_w=1.0, eik_w=1.0, mask_w=1.0,
-                  loss_type="l2"):
-    """All inputs torch tensors.  sdf_s [S, Ph, Pw, 1], sdf_e [Nd, Ph, Pw, 1] (ends of the
-    samples where diff_mask is true, in order), grad_s [S, Ph, Pw, 3] for 'ad' (None = dfd),
-    mask [B, Ph, Pw, 1] already binarised.  Returns (loss, parts, comp_normal, weight_sum)."""
-    B, Ph, Pw, _ = rays_o_all.shape
-    P = Ph * Pw
-    S = ri.shape[0]
-    dm = diff_mask(ri, t0, t1)
-    ts = expand_t(t0, ri, rays_d_all, plane_n)
-    te = expand_t(t1, ri, rays_d_all, plane_n)
-    # ends: shifted starts, replaced by their own values where dm
-    sdf_end = torch.cat([sdf_s[1:], sdf_s[-1:]], 0).clone()
-    sdf_end[dm] = sdf_e
+def neus_alpha(sdf_s, sdf_end, inv_s):
+    """NeuS discrete opacity (renderer.py:171-179):
+    clip((sigmoid(s f_i) - sigmoid(s f_i+1) + 1e-5) / (sigmoid(s f_i) + 1e-5), 0, 1)."""
     prev_cdf = torch.sigmoid(sdf_s * inv_s)
     next_cdf = torch.sigmoid(sdf_end * inv_s)
-    alpha = ((prev_cdf - next_cdf + 1e-5) / (prev_cdf + 1e-5)).clip(0.0, 1.0)
-    w = patch_weights(alpha.reshape(S, P), packed).reshape(S, Ph, Pw, 1)
-    if grad_s is None:
-        pos = positions(rays_o_all, rays_d_all, ri, ts).detach()
-        dist_x = torch.norm(pos[:, :, 1:] - pos[:, :, :-1], dim=-1, keepdim=True)
-        dist_y = torch.norm(pos[:, 1:] - pos[:, :-1], dim=-1, keepdim=True)
-        df_dt = (sdf_end - sdf_s) / (te - ts)
-        f = sdf_s
-        mid_x = (f[:, :, 2:] - f[:, :, :-2]) / (dist_x[:, :, :-1] + dist_x[:, :, 1:])
-        mid_y = (f[:, 2:] - f[:, :-2]) / (dist_y[:, 1:] + dist_y[:, :-1])
-        left = (f[:, :, 1:2] - f[:, :, 0:1]) / dist_x[:, :, 0:1]
-        right = (f[:, :, -1:] - f[:, :, -2:-1]) / dist_x[:, :, -1:]
-        top = (f[:, 1:2] - f[:, 0:1]) / dist_y[:, 0:1]
-        bottom = (f[:, -1:] - f[:, -2:-1]) / dist_y[:, -1:]
-        df_dx = torch.cat([left, mid_x, right], 2)
-        df_dy = torch.cat([top, mid_y, bottom], 1)
-        proj = torch.cat([df_dt, df_dx, df_dy], -1)
-        grads = (v_inv[ri] @ proj[..., None])[..., 0]
-    else:
-        grads = grad_s
-    ws = torch.zeros(B, Ph, Pw, 1, dtype=w.dtype).index_add(0, ri, w)
-    cn = torch.zeros(B, Ph, Pw, 3, dtype=w.dtype).index_add(0, ri, w * grads)
+    return ((prev_cdf - next_cdf + 1e-5) / (prev_cdf + 1e-5)).clip(0.0, 1.0)
+
+
+def dfd_gradients(rays_o_all, rays_d_all, ri, ts, te, sdf_s, sdf_end, v_inv):
+    """'dfd' SDF gradients (renderer.py:187-223): forward difference along each
+    patch ray, central differences across the patch on the marching plane
+    (one-sided at the border, distances between neighbouring start points),
+    then V^-1 [df/dt, df/dx, df/dy].  ts/te [S, Ph, Pw, 1]; -> [S, Ph, Pw, 3]."""
+    pos = positions(rays_o_all, rays_d_all, ri, ts).detach()
+    dist_x = torch.norm(pos[:, :, 1:] - pos[:, :, :-1], dim=-1, keepdim=True)
+    dist_y = torch.norm(pos[:, 1:] - pos[:, :-1], dim=-1, keepdim=True)
+    df_dt = (sdf_end - sdf_s) / (te - ts)
+    f = sdf_s
+    mid_x = (f[:, :, 2:] - f[:, :, :-2]) / (dist_x[:, :, :-1] + dist_x[:, :, 1:])
+    mid_y = (f[:, 2:] - f[:, :-2]) / (dist_y[:, 1:] + dist_y[:, :-1])
+    left = (f[:, :, 1:2] - f[:, :, 0:1]) / dist_x[:, :, 0:1]
+    right = (f[:, :, -1:] - f[:, :, -2:-1]) / dist_x[:, :, -1:]
+    top = (f[:, 1:2] - f[:, 0:1]) / dist_y[:, 0:1]
+    bottom = (f[:, -1:] - f[:, -2:-1]) / dist_y[:, -1:]
+    df_dx = torch.cat([left, mid_x, right], 2)
+    df_dy = torch.cat([top, mid_y, bottom], 1)
+    proj = torch.cat([df_dt, df_dx, df_dy], -1)
+    return (v_inv[ri] @ proj[..., None])[..., 0]
+
+
+def patch_losses(cn, ws, grads, n_true, mask, normal_w=1.0, eik_w=1.0, mask_w=1.0,
+                 loss_type="l2"):
+    """exp_runner.py:169-203: masked normal loss (sum of squared / absolute
+    errors over the mask's sum + 1e-5), eikonal mean((|grad| - 1)^2) over every
+    sample, BCE of the clipped weight sum against the mask.
+    Returns (loss, parts)."""
     mask_sum = mask.sum() + 1e-5
     err = (cn - n_true) * mask
     if loss_type == "l1":
@@ -112,4 +109,34 @@ def render_losses(rays_o_all, rays_d_all, plane_n, v_inv, n_true, mask, packed, 
     l_e = ((gn - 1.0) ** 2).mean()
     l_m = torch.nn.functional.binary_cross_entropy(ws.clip(1e-5, 1.0 - 1e-5), mask)
     loss = normal_w * l_n + mask_w * l_m + eik_w * l_e
-    return loss, dict(normal=normal_w * l_n, mask=mask_w * l_m, eikonal=eik_w * l_e), cn, ws
+    return loss, dict(normal=normal_w * l_n, mask=mask_w * l_m, eikonal=eik_w * l_e)
+
+
+def render_losses(rays_o_all, rays_d_all, plane_n, v_inv, n_true, mask, packed, ri, t0, t1,
+                  sdf_s, sdf_e, inv_s, grad_s=None, normal_w=1.0, eik_w=1.0, mask_w=1.0,
+                  loss_type="l2", out=None):
+    """All inputs torch tensors.  sdf_s [S, Ph, Pw, 1], sdf_e [Nd, Ph, Pw, 1] (ends of the
+    samples where diff_mask is true, in order), grad_s [S, Ph, Pw, 3] for 'ad' (None = dfd),
+    mask [B, Ph, Pw, 1] already binarised.  Returns (loss, parts, comp_normal, weight_sum);
+    `out` (a dict, optional) receives alpha, weights and gradients."""
+    B, Ph, Pw, _ = rays_o_all.shape
+    P = Ph * Pw
+    S = ri.shape[0]
+    dm = diff_mask(ri, t0, t1)
+    ts = expand_t(t0, ri, rays_d_all, plane_n)
+    te = expand_t(t1, ri, rays_d_all, plane_n)
+    # ends: shifted starts, replaced by their own values where dm
+    sdf_end = torch.cat([sdf_s[1:], sdf_s[-1:]], 0).clone()
+    sdf_end[dm] = sdf_e
+    alpha = neus_alpha(sdf_s, sdf_end, inv_s)
+    w = patch_weights(alpha.reshape(S, P), packed).reshape(S, Ph, Pw, 1)
+    if grad_s is None:
+        grads = dfd_gradients(rays_o_all, rays_d_all, ri, ts, te, sdf_s, sdf_end, v_inv)
+    else:
+        grads = grad_s
+    ws = torch.zeros(B, Ph, Pw, 1, dtype=w.dtype).index_add(0, ri, w)
+    cn = torch.zeros(B, Ph, Pw, 3, dtype=w.dtype).index_add(0, ri, w * grads)
+    loss, parts = patch_losses(cn, ws, grads, n_true, mask, normal_w, eik_w, mask_w, loss_type)
+    if out is not None:
+        out.update(alpha=alpha, weights=w, gradients=grads)
+    return loss, parts, cn, ws

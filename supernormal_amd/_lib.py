@@ -76,11 +76,12 @@ SIGNATURES = {
                                   _i32, _i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p,
                                   _p, _p, _p, _p]),
     "sn_mlp_pack_bwd": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p]),
-    "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p]),
+    "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p, _p]),
     "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
     "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),
     "sn_adam_step_copy16": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
-                                      C.c_double, _i32, _p, _p, _i64, _i64, _p]),
+                                      C.c_double, _i32, _p, _p, _i64, _i64, _p, _p]),
+    "sn_step_advance": (C.c_int, [_p, _p, _p, _p]),
     "sn_to_half": (C.c_int, [_p, _p, _i64, _p]),
     "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
@@ -105,7 +106,16 @@ _lib = None
 
 
 class SNError(RuntimeError):
-    """A failed call into libsupernormal_amd (message from sn_last_error)."""
+    """A failed call into libsupernormal_amd (message from sn_last_error).
+
+    A RuntimeError, as the reference's TORCH_CHECK / CHECK_INPUT failures in
+    its _C extension are (helpers_cuda.h:21-26): a bad argument (SN_EINVAL)
+    and a failed HIP call (SN_EHIP) raise the same type; `code` tells them
+    apart."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load() -> C.CDLL:
@@ -134,8 +144,8 @@ def check(rc: int, name: str) -> None:
     if rc != 0:
         msg = load().sn_last_error().decode(errors="replace")
         if rc == -1:
-            raise ValueError(f"{name}: {msg}")
-        raise SNError(f"{name} failed ({rc}): {msg}")
+            raise SNError(f"{name}: {msg}", rc)
+        raise SNError(f"{name} failed ({rc}): {msg}", rc)
 
 
 class LaunchTimer:

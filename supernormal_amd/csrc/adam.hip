@@ -29,7 +29,11 @@ __global__ void __launch_bounds__(256)
 adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
             float* __restrict__ v, int64_t n, AdamK k, float lr_scalar,
             const float* __restrict__ lr_dev, float bc1, const int32_t* __restrict__ step_dev,
-            _Float16* __restrict__ copy16, int64_t copy_off, int64_t copy_n) {
+            _Float16* __restrict__ copy16, int64_t copy_off, int64_t copy_n,
+            const int32_t* __restrict__ run_dev) {
+  // an iteration with no sample leaves parameters and moments as they are
+  // (the reference's iteration never reaches its optimiser step then)
+  if (run_dev && *run_dev <= 0) return;  // uniform
   if (step_dev) {  // bias corrections of the step held on the device
     const double st = (double)*step_dev;
     bc1 = (float)(1.0 - pow(k.b1d, st));
@@ -67,6 +71,16 @@ adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restric
   }
 }
 
+// Advances the device step count of an iteration that updates (run > 0) or
+// counts it as skipped.
+__global__ void step_advance_kernel(int32_t* __restrict__ step_dev,
+                                    const int32_t* __restrict__ run_dev,
+                                    int32_t* __restrict__ skipped_dev) {
+  if (threadIdx.x != 0) return;
+  if (*run_dev > 0) ++*step_dev;
+  else if (skipped_dev) ++*skipped_dev;
+}
+
 __global__ void __launch_bounds__(256)
 to_half_kernel(const float* __restrict__ src, _Float16* __restrict__ dst, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -83,13 +97,22 @@ extern "C" {
 int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                         int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                         double eps, int32_t step, const int32_t* step_dev, void* copy16,
-                        int64_t copy_offset, int64_t copy_n, void* stream);
+                        int64_t copy_offset, int64_t copy_n, const int32_t* run_dev,
+                        void* stream);
 
 int sn_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                  int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                  double eps, int32_t step, const int32_t* step_dev, void* stream) {
   return sn_adam_step_copy16(params, grads, exp_avg, exp_avg_sq, n, lr, lr_dev, beta1, beta2,
-                             eps, step, step_dev, nullptr, 0, 0, stream);
+                             eps, step, step_dev, nullptr, 0, 0, nullptr, stream);
+}
+
+int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_dev,
+                    void* stream) {
+  clear_error();
+  SN_REQUIRE(step_dev && run_dev, "sn_step_advance: null pointer");
+  step_advance_kernel<<<1, 64, 0, as_stream(stream)>>>(step_dev, run_dev, skipped_dev);
+  return check_launch("sn_step_advance");
 }
 
 int sn_to_half(const float* src, void* dst, int64_t n, void* stream) {
@@ -105,7 +128,8 @@ int sn_to_half(const float* src, void* dst, int64_t n, void* stream) {
 int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                         int64_t n, float lr, const float* lr_dev, double beta1, double beta2,
                         double eps, int32_t step, const int32_t* step_dev, void* copy16,
-                        int64_t copy_offset, int64_t copy_n, void* stream) {
+                        int64_t copy_offset, int64_t copy_n, const int32_t* run_dev,
+                        void* stream) {
   clear_error();
   SN_REQUIRE(!copy16 || (copy_offset >= 0 && copy_n >= 0 && copy_offset + copy_n <= n),
              "sn_adam_step: copy range [%lld, %lld) outside [0, %lld)", (long long)copy_offset,
@@ -132,7 +156,7 @@ int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float
   k.step_size = 0.f;
   adam_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, as_stream(stream)>>>(
       params, grads, exp_avg, exp_avg_sq, n, k, lr, lr_dev, (float)bc1, step_dev,
-      reinterpret_cast<_Float16*>(copy16), copy_offset, copy_n);
+      reinterpret_cast<_Float16*>(copy16), copy_offset, copy_n, run_dev);
   return check_launch("sn_adam_step");
 }
 

@@ -20,6 +20,6 @@ extern "C" {
 
 const char* sn_last_error(void) { return sn::g_err; }
 
-int sn_abi_version(void) { return 1; }
+int sn_abi_version(void) { return 2; }
 
 }  // extern "C"

@@ -634,15 +634,19 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       // ranks within the workgroup's share of each bucket
       const int gb0 = bd.base[l], nb = bd.base[l + 1] - gb0;
       int32_t rank[8];
-      float vmax = 0.f;
+      // largest |value| as bits: the bits of |x| order like |x| for finite
+      // values and inf, and every NaN's bits are above inf's, so a NaN or inf
+      // reaches the level maximum and the sums fall back to fp32 (fmaxf would
+      // drop a NaN)
+      uint32_t vmaxb = 0u;
       if (has) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           rank[c] = atomicAdd(&hist[idx[c] >> kBucketShift], 1);
 #pragma unroll
-          for (int f = 0; f < F; ++f) vmax = fmaxf(vmax, fabsf(rv[c][f]));
+          for (int f = 0; f < F; ++f) vmaxb = max(vmaxb, __float_as_uint(fabsf(rv[c][f])));
         }
-        atomicMax(&lmax, __float_as_uint(vmax));
+        atomicMax(&lmax, vmaxb);
       }
       if (gx && act) {
         float dw[8][3];

@@ -289,12 +289,13 @@ __global__ void __launch_bounds__(256)
 march_gather_kernel(const int32_t* __restrict__ packed_info, int32_t n_rays, int32_t slot,
                     const float* __restrict__ t0p, const float* __restrict__ t1p,
                     int64_t* __restrict__ ray_indices, float* __restrict__ t_starts,
-                    float* __restrict__ t_ends) {
+                    float* __restrict__ t_ends, int32_t* __restrict__ n_capped) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= (int64_t)n_rays * slot) return;
   const int32_t i = (int32_t)(g / slot), j = (int32_t)(g - (int64_t)i * slot);
   const int32_t base = packed_info[2 * i], cnt = packed_info[2 * i + 1];
   if (j >= cnt) return;
+  if (n_capped && j == slot - 1) atomicAdd(n_capped, 1);  // the ray filled its slots
   t_starts[base + j] = t0p[g];
   t_ends[base + j] = t1p[g];
   ray_indices[base + j] = i;
@@ -402,7 +403,7 @@ int sn_occ_pack_bricks(const uint8_t* binary, int32_t rx, int32_t ry, int32_t rz
 
 int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_ray,
                     const float* t_starts_pad, const float* t_ends_pad, int64_t* ray_indices,
-                    float* t_starts, float* t_ends, void* stream) {
+                    float* t_starts, float* t_ends, int32_t* n_capped, void* stream) {
   clear_error();
   SN_REQUIRE(n_rays >= 0 && max_per_ray > 0, "sn_march_gather: bad sizes");
   if (n_rays == 0) return SN_OK;
@@ -410,7 +411,8 @@ int sn_march_gather(const int32_t* packed_info, int32_t n_rays, int32_t max_per_
              "sn_march_gather: null pointer");
   const int64_t total = (int64_t)n_rays * max_per_ray;
   march_gather_kernel<<<(unsigned)ceil_div(total, 256), 256, 0, as_stream(stream)>>>(
-      packed_info, n_rays, max_per_ray, t_starts_pad, t_ends_pad, ray_indices, t_starts, t_ends);
+      packed_info, n_rays, max_per_ray, t_starts_pad, t_ends_pad, ray_indices, t_starts, t_ends,
+      n_capped);
   return check_launch("sn_march_gather");
 }
 

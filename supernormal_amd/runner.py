@@ -157,6 +157,10 @@ class Runner:
             self.iter_step += 1
             self.update_learning_rate()
             return losses[:3].sum()
+        if self.sf_step is not None:
+            # leaving the device iteration: its step count is the optimiser's
+            self.sf_step.drain()
+            self.sf_step.sync_step_count()
         o, d, plane_n, v_inv, n_true, mask = self.dataset.gen_random_patches(
             self.batch_size, patch_H=self.patch_size, patch_W=self.patch_size)
         pc = self.patch_size // 2
@@ -238,6 +242,8 @@ class Runner:
         return os.path.join(self.base_exp_dir, "checkpoints", f"ckpt_{it:06d}.pth")
 
     def save_checkpoint(self):
+        if self.sf_step is not None:
+            self.sf_step.sync_step_count()
         os.makedirs(os.path.join(self.base_exp_dir, "checkpoints"), exist_ok=True)
         ck = {
             "sdf_network_fine": self.sdf_network.state_dict(),

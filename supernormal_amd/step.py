@@ -147,10 +147,18 @@ class SyncFreeStep:
         self.adam_t = torch.zeros(1, **i32)
         self.levels_dev = torch.zeros(1, **i32)  # active hash levels (bandwidth)
         self.inv_s = torch.zeros(1, **f32)
+        # device counts of one iteration, views of one buffer: marched centre
+        # samples, visible ones, centre points and patch points evaluated; then
+        # two running counters: rays that filled their sample capacity and
+        # iterations skipped for want of samples
+        self.stat = torch.zeros(8, **i32)
+        self.n0, self.ns, self.ne0, self.ne = (self.stat[k:k + 1] for k in range(4))
+        self.n_capped, self.n_skipped = self.stat[4:5], self.stat[5:6]
+        # the four per-iteration counts summed over iterations (int64)
+        self.stat_acc = torch.zeros(4, dtype=torch.int64, device=dev)
         # marching
         self.counts = torch.zeros(B, **i32)
         self.packed0 = torch.zeros(B, 2, **i32)
-        self.n0 = torch.zeros(1, **i32)
         self.ri0 = torch.zeros(c0, dtype=torch.int64, device=dev)
         self.t00 = torch.zeros(c0, **f32)
         self.t10 = torch.zeros(c0, **f32)
@@ -171,7 +179,6 @@ class SyncFreeStep:
         # visibility pre-pass
         self.didx0 = torch.zeros(c0, **i32)
         self.nd0 = torch.zeros(1, **i32)
-        self.ne0 = torch.zeros(1, **i32)
         self.pts0 = torch.zeros(2 * c0, 3, **f32)
         self.enc0 = torch.zeros(2 * c0, E, **f32)
         self.sdf0 = torch.zeros(2 * c0, **f32)
@@ -180,11 +187,9 @@ class SyncFreeStep:
         self.ri = torch.zeros(c0, dtype=torch.int64, device=dev)
         self.t0 = torch.zeros(c0, **f32)
         self.t1 = torch.zeros(c0, **f32)
-        self.ns = torch.zeros(1, **i32)
         self.packed = torch.zeros(B, 2, **i32)
         self.didx = torch.zeros(c0, **i32)
         self.nd = torch.zeros(1, **i32)
-        self.ne = torch.zeros(1, **i32)
         self.pts = torch.zeros(self.c1, 3, **f32)
         self.enc = torch.zeros(self.c1, E, **f32)
         self.sdf = torch.zeros(self.c1, **f32)
@@ -338,7 +343,8 @@ class SyncFreeStep:
         call("sn_counts_to_packed_info", ptr(self.counts), B, ptr(self.packed0), ptr(self.n0),
              ptr(self.ws_scan), self.ws_scan.numel(), s)
         call("sn_march_gather", ptr(self.packed0), B, self.msr, ptr(self.t0_pad),
-             ptr(self.t1_pad), ptr(self.ri0), ptr(self.t00), ptr(self.t10), s)
+             ptr(self.t1_pad), ptr(self.ri0), ptr(self.t00), ptr(self.t10),
+             ptr(self.n_capped), s)
 
     def _part_rest(self):
         """From the marched samples up to the hash-grid backward: the visibility
@@ -392,6 +398,9 @@ class SyncFreeStep:
         call("sn_mlp_pack_bwd", ptr(f.data), self.n_pack, self.pack_layers.ctypes.data,
              ptr(self.g_mlp), ptr(f.grad), ptr(self.var_data), ptr(self.losses[3:4]),
              ptr(self.var_grad), s)
+        # this iteration's counts (the march of the next one may run ahead once
+        # this part is done, and it rewrites n0)
+        self.stat_acc.add_(self.stat[:4])
 
     def _part_hash(self):
         """Hash-grid backward into the flat gradient buffer (bucket records,
@@ -403,15 +412,16 @@ class SyncFreeStep:
     def _part_adam(self):
         """Adam with the learning rate and step count on the device."""
         s = stream_of(self.dev)
-        self.adam_t += 1
+        # an iteration with no visible sample is skipped (parameters, moments and
+        # step count unchanged; the reference raises KeyError there,
+        # exp_runner.py:182) and counted in n_skipped
+        call("sn_step_advance", ptr(self.adam_t), ptr(self.ns), ptr(self.n_skipped), s)
         f = self.flat
-        if self.table16 is not None:
-            call("sn_adam_step_copy16", ptr(f.data), ptr(f.grad), ptr(f.exp_avg),
-                 ptr(f.exp_avg_sq), f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0,
-                 ptr(self.adam_t), ptr(self.table16), self.table_off, self.table16.numel(), s)
-        else:
-            call("sn_adam_step", ptr(f.data), ptr(f.grad), ptr(f.exp_avg), ptr(f.exp_avg_sq),
-                 f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0, ptr(self.adam_t), s)
+        t16 = self.table16
+        call("sn_adam_step_copy16", ptr(f.data), ptr(f.grad), ptr(f.exp_avg),
+             ptr(f.exp_avg_sq), f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0,
+             ptr(self.adam_t), ptr(t16), self.table_off if t16 is not None else 0,
+             t16.numel() if t16 is not None else 0, ptr(self.ns), s)
 
     # ------------------------------------------------------------------- driver
     def step(self, lr, step_size, use_graph=True, inputs=None, next_step_size=None):
@@ -499,6 +509,30 @@ class SyncFreeStep:
             ev = torch.cuda.Event()
             ev.record(self.side)
         self._pending = (ev, step_size)
+
+    def sync_step_count(self):
+        """Sets the host's optimiser step count (flat.step_count) to the device's,
+        which does not advance on skipped iterations; reads device memory."""
+        t = int(self.adam_t)
+        self.flat.step_count = t
+        self._adam_t_host = t
+        return t
+
+    @property
+    def skipped_iterations(self):
+        """Iterations skipped so far for want of visible samples (reads the device)."""
+        return int(self.n_skipped)
+
+    @property
+    def capped_rays(self):
+        """Centre rays so far whose samples filled max_samples_per_ray (the cap may
+        have truncated them; reads the device)."""
+        return int(self.n_capped)
+
+    def counts_total(self):
+        """Sums over the iterations so far of (marched centre samples, visible
+        centre samples, centre points, patch points evaluated)."""
+        return [int(v) for v in self.stat_acc.tolist()]
 
     @property
     def march_pending(self):

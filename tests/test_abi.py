@@ -46,6 +46,8 @@ def test_error_reporting_without_gpu():
     rc = lib.sn_ray_march_count(None, None, None, None, 4, None, None, 8, 8, 8,
                                 ctypes.c_float(0.0), None, ctypes.c_float(0.0), 0, None, None)
     assert rc == -1 and b"render_step_size" in lib.sn_last_error()
-    with pytest.raises(ValueError):
+    # a bad argument raises RuntimeError, as the reference's TORCH_CHECK does
+    with pytest.raises(RuntimeError) as e:
         _lib.check(lib.sn_hashgrid_fwd(None, 10, None, 0, 2, 19, 16, ctypes.c_float(1.5), 0,
                                        None, None), "sn_hashgrid_fwd")
+    assert e.value.code == -1

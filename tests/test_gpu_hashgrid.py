@@ -273,7 +273,7 @@ def test_forward_from_fp16_copy(name, n_active, dev):
     grad = torch.randn(m + 1003, generator=g).to(dev)
     mom1, mom2 = torch.zeros_like(flat), torch.zeros_like(flat)
     call("sn_adam_step_copy16", ptr(flat), ptr(grad), ptr(mom1), ptr(mom2), flat.numel(), 1e-2,
-         None, 0.9, 0.999, 1e-8, 1, None, ptr(t16), 1001, m, st)
+         None, 0.9, 0.999, 1e-8, 1, None, ptr(t16), 1001, m, None, st)
     assert torch.equal(t16, flat[1001:1001 + m].half())
 
 

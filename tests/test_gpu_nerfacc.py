@@ -255,11 +255,15 @@ def test_ray_marching_padded_bitexact(dev, res, step, cap, roi_name, bricks):
     ri = torch.empty(n * cap, dtype=torch.int64, device=dev)
     t0 = torch.empty(n * cap, device=dev)
     t1 = torch.empty(n * cap, device=dev)
-    call("sn_march_gather", ptr(packed), n, cap, ptr(t0p), ptr(t1p), ptr(ri), ptr(t0), ptr(t1), st)
+    n_capped = torch.zeros(1, dtype=torch.int32, device=dev)
+    call("sn_march_gather", ptr(packed), n, cap, ptr(t0p), ptr(t1p), ptr(ri), ptr(t0), ptr(t1),
+         ptr(n_capped), st)
     m = int(total)
     pi_o, ri_o, t0_o, t1_o = c_oracle.ray_marching(o, d, near, far, roi, grid, step,
                                                    max_per_ray=cap)
     assert m == ri_o.size and m > 1000
+    # rays whose count reached the cap (the ones it may have truncated)
+    assert int(n_capped) == int((pi_o[:, 1] == cap).sum())
     np.testing.assert_array_equal(packed.cpu().numpy(), pi_o)
     np.testing.assert_array_equal(ri[:m].cpu().numpy(), ri_o)
     np.testing.assert_array_equal(t0[:m].cpu().numpy(), t0_o[:, 0] if t0_o.ndim == 2 else t0_o)

@@ -1,5 +1,6 @@
 """Pins the CPU oracle against the reference's own golden vectors, closed forms,
 invariants and brute force.  CPU only."""
+import math
 import numpy as np
 import pytest
 import torch
@@ -200,6 +201,50 @@ def test_march_samples_in_occupied_cells_and_cap_is_prefix():
     for r in range(128):
         np.testing.assert_array_equal(t0c[pic[r, 0]:pic[r, 0] + pic[r, 1]],
                                       t0[pi[r, 0]:pi[r, 0] + pic[r, 1]])
+
+
+def _slab_march(near, step, far=2.0):
+    """Ray along +x from x = -1 (t = x + 1) through an 8^3 grid over the roi
+    [-1, 1]^3 whose only occupied cells are the slab ix = 5 (x in [0.25, 0.5),
+    t in [1.25, 1.5)); y = z = 0.15 so the DDA's y / z terms are +inf."""
+    o = np.array([[-1.0, 0.15, 0.15]], np.float32)
+    d = np.array([[1.0, 0.0, 0.0]], np.float32)
+    grid = np.zeros((8, 8, 8), np.uint8)
+    grid[5] = 1
+    return c_oracle.ray_marching(o, d, np.array([near], np.float32),
+                                 np.array([far], np.float32), ROI, grid, step)
+
+
+def test_march_skip_then_slab_aligned():
+    """Closed form of the empty-space skip (ray_marching.cu:47-75, 165-171) with
+    dyadic numbers: every skip moves t_mid by a whole cell (4 steps of 1/16), so
+    t_mid = 1/32 + k/4 lands at 1.28125 in the slab, the interval is re-centred
+    on it (t0 = t_mid - dt/2 = 1.25, the slab's boundary) and the slab's four
+    steps are emitted contiguously; the skips after the slab end the ray."""
+    pi, ri, t0, t1 = _slab_march(0.0, 0.0625)
+    assert pi.tolist() == [[0, 4]] and ri.tolist() == [0, 0, 0, 0]
+    np.testing.assert_array_equal(t0[:, 0], [1.25, 1.3125, 1.375, 1.4375])
+    np.testing.assert_array_equal(t1[:, 0], [1.3125, 1.375, 1.4375, 1.5])
+
+
+def test_march_skip_then_slab_jittered():
+    """With a stratified start the skip does not re-align to the cell grid: the
+    first sample's midpoint is near + dt/2 + k*dt with the smallest k that puts
+    it at or past the slab boundary t = 1.25 (advance_to_next_voxel steps by
+    dt_min from t_mid, ray_marching.cu:62-75); then the slab is sampled every dt
+    while the midpoint stays inside, t0 / t1 = midpoint -/+ dt/2."""
+    step = 0.0625
+    for near in (0.01, 0.02, 0.037, 0.05):
+        pi, ri, t0, t1 = _slab_march(near, step)
+        m0 = near + step / 2
+        k = math.ceil((1.25 - m0) / step)
+        mids = [m0 + (k + j) * step for j in range(8)]
+        mids = [m for m in mids if m < 1.5]
+        assert int(pi[0, 1]) == len(mids) == ri.size
+        np.testing.assert_allclose((t0[:, 0] + t1[:, 0]) / 2, mids, atol=2e-6)
+        np.testing.assert_allclose(t1[:, 0] - t0[:, 0], step, atol=2e-6)
+        # contiguous inside the slab
+        np.testing.assert_array_equal(t0[1:, 0], t1[:-1, 0])
 
 
 # ---------------------------------------------------------------- occupancy
