@@ -316,6 +316,8 @@ class OccupancyGrid(nn.Module):
         # extension: a torch.Generator for the update's cell sampling and jitter
         # (None = torch's default generator, the reference's behaviour)
         self.generator = None
+        # extension: cells evaluated by the updates so far (a host count)
+        self.points_evaluated = 0
 
     @property
     def roi_aabb(self) -> Tensor:
@@ -369,6 +371,20 @@ class OccupancyGrid(nn.Module):
             indices = self._sample_uniform_and_occupied_cells(self.num_cells // 4)
         n = indices.numel()
         jitter = torch.rand((n, 3), device=dev, dtype=torch.float32, generator=self.generator)
+        self.update_cells(indices, jitter, occ_eval_fn, occ_thre, ema_decay)
+
+    @torch.no_grad()
+    def update_cells(self, indices: Tensor, jitter: Tensor, occ_eval_fn: Callable,
+                     occ_thre: float = 0.01, ema_decay: float = 0.95) -> None:
+        """The update of the given cells with the given jitter in [0,1)^3
+        (grid.py:214-239): points in the cells, occ_eval_fn, EMA by scatter-max,
+        binarisation by min(mean, occ_thre).  Extension: _update draws the cells
+        and the jitter, then calls this."""
+        dev = self.device
+        indices = indices.to(device=dev, dtype=torch.int64).contiguous()
+        jitter = jitter.to(device=dev, dtype=torch.float32).contiguous()
+        n = indices.numel()
+        self.points_evaluated += n
         x = torch.empty((n, 3), device=dev, dtype=torch.float32)
         rx, ry, rz = self._res
         s = stream_of(dev)

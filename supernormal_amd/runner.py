@@ -55,8 +55,7 @@ class Runner:
         torch.manual_seed(seed)
         self.base_exp_dir = base_exp_dir or c.get("general.base_exp_dir", "./exp/run")
         if dataset is None:
-            from .scene import config_dataset
-            dataset = config_dataset(c.get("dataset.synthetic", "sphere128"), self.device, seed)
+            dataset = self._config_dataset(c, seed, datadir)
         self.dataset = dataset
         self.iter_step = 0
         self.end_iter = c.get_int("train.end_iter")
@@ -115,6 +114,28 @@ class Runner:
         self.history = []
         if is_continue:
             self.load_checkpoint()
+
+    def _config_dataset(self, c, seed, datadir=None):
+        """The dataset the config names (exp_runner.py:45-48): `dataset.synthetic`
+        (one of supernormal_amd.scene's workloads) or the reference's on-disk
+        format (`dataset.data_dir` etc., supernormal_amd.dataset_loader), with
+        `general.dataset_class` naming the reference's loader when given."""
+        ds_conf = c.get("dataset")
+        if not isinstance(ds_conf, dict):
+            raise KeyError("the config has no dataset block")
+        cls = c.get("general.dataset_class")
+        if "synthetic" in ds_conf:
+            from .scene import config_dataset
+            return config_dataset(ds_conf["synthetic"], self.device, seed)
+        if cls not in (None, "models.dataset_loader.Dataset",
+                       "supernormal_amd.dataset_loader.Dataset"):
+            raise ValueError(f"general.dataset_class {cls!r} is not supported (the reference's "
+                             "models.dataset_loader.Dataset format or dataset.synthetic)")
+        from .dataset_loader import load_dataset
+        ds_conf = Conf(ds_conf)
+        if datadir is not None:
+            ds_conf["data_dir"] = datadir
+        return load_dataset(ds_conf, self.device)
 
     def _make_sync_free_step(self):
         from .step import SyncFreeStep
@@ -287,7 +308,7 @@ class Runner:
             self._make_sync_free_step()
 
 
-def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=5000, lr=None,
+def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=None, lr=None,
                    batch_size=None, max_samples_per_ray=None, mlp_precision="fp32",
                    step_mode="fused") -> Conf:
     """The synthetic training workloads as configs with the reference's keys.
@@ -342,3 +363,40 @@ def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=5000, lr=N
                 train=Conf(train), val=Conf(report_freq=100, save_freq=10 ** 9),
                 model=Conf(sdf_network=Conf(net), variance_network=Conf(init_val=0.5),
                            ray_marching=Conf(rm), encoding=Conf(enc)))
+
+
+def build_parser():
+    """The command line of exp_runner.py:639-645 (same flags and defaults)."""
+    import argparse
+    p = argparse.ArgumentParser(description="SuperNormal training on supernormal_amd")
+    p.add_argument("--conf", type=str, default="./confs/base.conf")
+    p.add_argument("--mode", type=str, default="eval_normal")
+    p.add_argument("--mcube_threshold", type=float, default=0.0)
+    p.add_argument("--is_continue", default=False, action="store_true")
+    p.add_argument("--gpu", type=int, default=0)
+    p.add_argument("--obj_name", type=str, default="")
+    return p
+
+
+def read_conf_text(path, obj_name):
+    """The config file's text with CASE_NAME replaced by the object's name
+    (exp_runner.py:651-653)."""
+    with open(path) as f:
+        return f.read().replace("CASE_NAME", obj_name)
+
+
+def main(argv=None):
+    """exp_runner.py:633-657: parse the flags, select the GPU, build the
+    Runner from the config file and train."""
+    args = build_parser().parse_args(argv)
+    torch.cuda.set_device(args.gpu)
+    print(f"Running on the object: {args.obj_name}")
+    conf_text = read_conf_text(args.conf, args.obj_name)
+    runner = Runner(conf_text, args.mode, args.is_continue, device=f"cuda:{args.gpu}")
+    runner.train()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

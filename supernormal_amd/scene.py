@@ -213,13 +213,13 @@ class Dataset:
     """Multi-view normal maps + masks of an analytic scene, in the reference's
     Dataset interface (models/dataset_loader.py)."""
 
-    def __init__(self, scene: Scene, n_views: int, W: int, H: int, noise_std: float,
+    def __init__(self, scene: Optional[Scene], n_views: int, W: int, H: int, noise_std: float,
                  device, seed: int = 0, elevations=(0.0,), radius: float = 2.5,
                  fov_deg: float = 35.0, exclude_views=()):
         self.device = torch.device(device)
         self.scene = scene
-        self.W, self.H = int(W), int(H)
-        self.n_images = n_views
+        if scene is None:  # filled by from_arrays
+            return
         K = intrinsics(W, H, fov_deg)
         poses = ring_cameras(n_views, radius, elevations, seed)
         g = torch.Generator(device="cpu").manual_seed(seed)
@@ -233,10 +233,26 @@ class Dataset:
                 n = torch.where(m[..., None], n, torch.zeros_like(n))
             normals.append(n.float())
             masks.append(m.float())
-        self.normals = torch.stack(normals)                        # [n,H,W,3]
-        self.masks = torch.stack(masks)                            # [n,H,W]
-        self.intrinsics_all = torch.as_tensor(np.stack([K] * n_views), dtype=torch.float32,
-                                              device=self.device)
+        self._setup(torch.stack(normals), torch.stack(masks), np.stack([K] * n_views),
+                    np.asarray(poses), exclude_views)
+
+    @classmethod
+    def from_arrays(cls, normals, masks, intrinsics_all, pose_all, device, exclude_views=()):
+        """A dataset of given views: normals [n,H,W,3], masks [n,H,W] (0 / 1),
+        intrinsics_all / pose_all [n,4,4] (camera-to-world pose)."""
+        ds = cls(None, 0, 0, 0, 0.0, device)
+        ds._setup(torch.as_tensor(normals, dtype=torch.float32, device=ds.device),
+                  torch.as_tensor(masks, dtype=torch.float32, device=ds.device),
+                  np.asarray(intrinsics_all), np.asarray(pose_all), exclude_views)
+        return ds
+
+    def _setup(self, normals, masks, K_all, poses, exclude_views):
+        n_views, H, W = normals.shape[:3]
+        self.W, self.H = int(W), int(H)
+        self.n_images = n_views
+        self.normals = normals.to(self.device)                     # [n,H,W,3]
+        self.masks = masks.to(self.device)                         # [n,H,W]
+        self.intrinsics_all = torch.as_tensor(K_all, dtype=torch.float32, device=self.device)
         self.intrinsics_all_inv = torch.inverse(self.intrinsics_all)
         self.pose_all = torch.as_tensor(poses, dtype=torch.float32, device=self.device)
         self.exclude_view_list = list(exclude_views)

@@ -105,6 +105,7 @@ class SyncFreeStep:
         self._adam_t_host = int(flat.step_count)
         self._levels_host = None
         self.graph = None
+        self.graph_given = None  # the march alone, for iterations on given inputs
         # the next iteration's march when it ran ahead: (event, step size)
         self._pending = None
         self.side = None
@@ -268,7 +269,10 @@ class SyncFreeStep:
              ptr(self.n_true), ptr(self.mask), ptr(self.mask_sum), ptr(self.o_c),
              ptr(self.d_c), ptr(self.t_min), ptr(self.t_max), stream_of(dev))
 
-    def _load(self, o, d, pn, vi, nt, m, stratified):
+    def _load(self, o, d, pn, vi, nt, m, stratified, t_min=None, t_max=None):
+        """Given patches into the static input buffers; t_min / t_max of the
+        centre rays when given (e.g. an already jittered start), else near / far
+        on the unit sphere (+ the stratified jitter)."""
         ds = self.dataset
         self.rays_o.copy_(o)
         self.rays_d.copy_(d)
@@ -281,6 +285,10 @@ class SyncFreeStep:
         kc = (self.Ph // 2, self.Pw // 2)
         self.o_c.copy_(self.rays_o[:, kc[0], kc[1]])
         self.d_c.copy_(self.rays_d[:, kc[0], kc[1]])
+        if t_min is not None:
+            self.t_min.copy_(t_min)
+            self.t_max.copy_(t_max)
+            return
         near, far = ds.near_far_from_sphere(self.o_c, self.d_c)
         # stratified start (renderer.py:69-71)
         if stratified:
@@ -427,15 +435,18 @@ class SyncFreeStep:
     def step(self, lr, step_size, use_graph=True, inputs=None, next_step_size=None):
         """One training iteration; lr / step_size for this iteration.  Returns
         the device tensor of losses (normal, mask, eikonal, dL/dinv_s).
-        `inputs` = (rays_o, rays_d, plane_n, V_inv, normals, mask) of fixed
-        patches with unjittered starts instead of sampling, or "keep" to rerun
-        on the inputs of the previous iteration (eager only).
+        `inputs` = (rays_o, rays_d, plane_n, V_inv, normals, mask[, t_min,
+        t_max]) of given patches instead of sampling (without t_min / t_max:
+        unjittered starts on the unit sphere), or "keep" to rerun on the inputs
+        of the previous iteration (eager only).
 
         Graph mode with `next_step_size`: the next iteration's sampling and march
         are launched on a side stream once this iteration no longer reads their
         buffers, so they overlap this iteration's hash-grid backward and Adam
         step.  The caller must not change the occupancy grid before the next
         step (the march has read it) and passes the same step size to it."""
+        if inputs is not None and not isinstance(inputs, str) and use_graph:
+            return self._step_given(lr, step_size, inputs)
         self.lr.fill_(float(lr))
         levels = int(self.net.active_levels())
         if levels != self._levels_host:
@@ -492,6 +503,42 @@ class SyncFreeStep:
         # not wait for the host to submit the (many-node) sampling graph
         if done is not None:
             self._march_ahead(done, float(next_step_size))
+        self.flat.step_count += 1
+        self._adam_t_host = t + 1
+        return self.losses
+
+    def _step_given(self, lr, step_size, inputs):
+        """Graph mode on given inputs: they are copied into the static buffers,
+        then a graph holding the march alone (no sampling) and the iteration's
+        other three graphs are replayed."""
+        if self._pending is not None:
+            raise RuntimeError("an iteration's march is pending; run it in graph mode")
+        self.lr.fill_(float(lr))
+        levels = int(self.net.active_levels())
+        if levels != self._levels_host:
+            self.levels_dev.fill_(levels)
+            self._levels_host = levels
+        t = int(self.flat.step_count)
+        if t != self._adam_t_host:
+            self.adam_t.fill_(t)
+        self._adam_t_host = None
+        if self.graph is None:
+            self._capture()
+        if self.graph_given is None:
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize(self.dev)
+            with torch.cuda.graph(g):
+                self._part_march("keep")
+            self.graph_given = g
+        self.step_size.fill_(float(step_size))
+        self._load(*inputs, stratified=False)
+        self.graph_given.replay()
+        self.graph[1].replay()
+        e = self._hash_events() if self.time_hash_bwd else None
+        self.graph[2].replay()
+        if e is not None:
+            e[1].record()
+        self.graph[3].replay()
         self.flat.step_count += 1
         self._adam_t_host = t + 1
         return self.losses
