@@ -193,13 +193,26 @@ def run_schedule(dev, workload, precision, method, mode, seed, W, K, max_samples
     return runner, res
 
 
-def accuracy(runner):
-    from supernormal_amd.mesh import chamfer_distance_and_f1_score, visible_surface_points
-    net = runner.sdf_network
-    pts = visible_surface_points(lambda x: net.fused_sdf(x), runner.dataset, 4000)
-    gt = runner.dataset.surface_points(4000 * len(runner.dataset.train_images))
+def accuracy(runner, resolution=512, rays_per_view=4000, net=None):
+    """Chamfer distance / F-score@0.005 of a trained field the reference's way
+    (exp_runner.py:579-630): marching cubes of the SDF on a resolution^3 grid
+    of the object's bounding box (renderer.py:27-34), first hits of masked
+    training-pixel rays with that mesh, against the true surface's first hits
+    along the training pixels (cd_and_fscore.py, exact fp64 nearest
+    neighbours)."""
+    from supernormal_amd.mesh import (chamfer_distance_and_f1_score, extract_fields_device,
+                                      marching_cubes_device, visible_mesh_points)
+    net = net or runner.sdf_network
+    ds = runner.dataset
+    bmin, bmax = list(ds.object_bbox_min), list(ds.object_bbox_max)
+    u = extract_fields_device(bmin, bmax, resolution, lambda x: -net.fused_sdf(x), ds.device)
+    mesh = marching_cubes_device(-u, 0.0, bmin, bmax, vertex_ids=False)
+    del u
+    pts = visible_mesh_points(mesh, ds, rays_per_view=rays_per_view)
+    gt = ds.surface_points(rays_per_view * len(ds.train_images))
     cd, fs = chamfer_distance_and_f1_score(gt, pts, f_threshold=0.005)
-    return {"chamfer": round(cd, 7), "fscore@0.005": round(fs, 4), "n_eval_points": int(len(pts))}
+    return {"chamfer": round(cd, 7), "fscore@0.005": round(fs, 4), "n_eval_points": int(len(pts)),
+            "mesh_triangles": int(mesh.tris.shape[0]), "mesh_resolution": resolution}
 
 
 def throughput_fields(runner, res, precision):
@@ -254,11 +267,19 @@ def roofline(res, grid):
                                "bytes_per_point": round(t["bytes_per_point"], 1)}}
 
 
+def cpu_threads():
+    """Host threads for the CPU legs: the cores this process may run on, capped
+    by OMP_NUM_THREADS when it is set (the host share of one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
+
+
 def cpu_sample(runner, iters=2):
     """The CPU oracle's iteration at this workload's size from the trained
     state: `iters` timed iterations after one untimed, median."""
     from oracle.train_oracle import OracleTrainer
-    threads = len(os.sched_getaffinity(0))
+    threads = cpu_threads()
     torch.set_num_threads(threads)
     ds = runner.dataset
     tr = OracleTrainer(runner.sdf_network, float(runner.deviation_network.variance.detach()),
@@ -285,62 +306,123 @@ def cpu_sample(runner, iters=2):
                       f"trained state of the same workload"}
 
 
-def cpu_small_workload(dev, seed=0):
-    """The small CPU workload (sphere r 0.5, 8 views 128^2, hash 8x2 T 2^15, MLP
-    32x1, 256 patches, occupancy 64^3, <= 32 samples/ray, 200 iterations, lr
-    1e-2): the CPU oracle trained whole, iteration time = median of iterations
-    5..24; the device path (graph mode, fp32 MLP) on the same workload beside it."""
+def cpu_small_workload(seed=0):
+    """The plain method on the CPU (torch fp32 restatement of the reference
+    iteration) on the small workload (sphere r 0.5, 8 views 128^2, hash 8x2 T
+    2^15, MLP 32x1, 256 patches, occupancy 64^3, <= 32 samples/ray, lr 1e-2):
+    median time of iterations 5..24 (after 5 warm-up)."""
     from oracle.train_oracle import OracleTrainer
-    from supernormal_amd.runner import Runner, synthetic_conf
-    threads = len(os.sched_getaffinity(0))
+    from supernormal_amd.fields import SDFNetwork, SingleVarianceNetwork
+    from supernormal_amd.runner import synthetic_conf
+    from supernormal_amd.scene import config_dataset
+    threads = cpu_threads()
     torch.set_num_threads(threads)
-    conf = synthetic_conf("sphere128", mlp_precision="fp32", step_mode="graph")
-    r = Runner(conf, fused=True, device=dev, seed=seed)
-    tr = OracleTrainer(r.sdf_network, float(r.deviation_network.variance.detach()),
-                       r.occ_resolution, {"lr": r.learning_rate, "loss_type": r.loss_type})
-    ds = r.dataset
-    g = torch.Generator(device=dev).manual_seed(seed + 99)
-    times, E = [], r.end_iter
-    it_step = lambda i: 10 ** (np.log10(r.start_step_size) - r.slop_step * i)
-    t_all = time.perf_counter()
-    for i in range(E):
-        tr.occupancy_update(i, n=r.occ_update_freq, thre=r.occ_threshold)
-        if i % r.increase_bindwidth_every == 0:  # exp_runner.py:158-159
+    conf = synthetic_conf("sphere128")
+    torch.manual_seed(seed)
+    net = SDFNetwork(**conf["model.sdf_network"], encoding_config=conf["model.encoding"])
+    var = SingleVarianceNetwork(**conf["model.variance_network"])
+    ds = config_dataset("sphere128", torch.device("cpu"), seed)
+    tr = OracleTrainer(net, float(var.variance.detach()), 64,
+                       {"lr": 1e-2, "loss_type": "l2"})
+    g = torch.Generator().manual_seed(seed + 99)
+    rm = conf["model.ray_marching"]
+    start, slope = rm["start_step_size"], \
+        (np.log10(rm["start_step_size"]) - np.log10(rm["end_step_size"])) / 200
+    times = []
+    for i in range(25):
+        tr.occupancy_update(i, n=8, thre=0.1)
+        if i % 25 == 0:
             tr.field.bindwidth += 1
-        o, d, pn, vi, nt, m = ds.gen_random_patches(r.batch_size, generator=g)
+        o, d, pn, vi, nt, m = ds.gen_random_patches(256, generator=g)
         near, far = ds.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
-        cpu = [t.cpu() for t in (o, d, pn, vi, nt, m, near, far)]
-        r.iter_step = i
-        r.update_learning_rate()
         t0 = time.perf_counter()
-        tr.step(*cpu, step_size=float(it_step(i)), lr=r.lr, cap=r.max_samples_per_ray)
+        tr.step(o, d, pn, vi, nt, m, near, far, step_size=float(10 ** (np.log10(start) - slope * i)),
+                lr=1e-2, cap=32)
         times.append(time.perf_counter() - t0)
-    cpu_total = time.perf_counter() - t_all
     sec = float(np.median(times[5:25]))
-    # the CPU-trained field's surface, evaluated with the device kernels
-    net = r.sdf_network
-    with torch.no_grad():
-        net.encoding.params.copy_(tr.field.params.detach())
-        for l in range(net.num_layers - 1):
-            lin = getattr(net, f"lin{l}")
-            lin.weight_v.copy_(tr.field.v[l].detach())
-            lin.weight_g.copy_(tr.field.g[l].detach())
-            lin.bias.copy_(tr.field.b[l].detach())
-    net.bindwidth = tr.field.bindwidth
-    acc_cpu = accuracy(r)
-    # the device path on the same workload (fresh runner, same seed)
-    r2, res2 = run_schedule(dev, "sphere128", "fp32", "dfd", "graph", seed, 5, 20, None,
-                            time_hash=False)
-    acc_dev = accuracy(r2)
     return {"value": round(1.0 / sec, 3), "unit": "train iters/s", "cores": threads,
             "kind": "port",
-            "sample": "small CPU workload: sphere r 0.5, 8 views 128^2, hash 8x2 T 2^15, MLP "
-                      "32x1, 256 patches, occupancy 64^3, <= 32 samples/ray; median iteration "
-                      "time of iterations 5..24 of the 200 trained",
-            "full_run_s": round(cpu_total, 2), "accuracy": acc_cpu,
-            "device_same_workload": {"value": round(res2["value"], 2),
-                                     "full_schedule": round(res2["full_schedule"], 2),
-                                     "accuracy": acc_dev}}
+            "sample": "small CPU workload (plain method, torch fp32): sphere r 0.5, 8 views "
+                      "128^2, hash 8x2 T 2^15, MLP 32x1, 256 patches, occupancy 64^3, <= 32 "
+                      "samples/ray; median iteration time of iterations 5..24 after 5 warm-up "
+                      "(occupancy updates outside the timed calls)"}
+
+
+def fp64_gate(dev, seed=0, precisions=("fp16", "fp32"), resolution=256):
+    """The device path against the FP64 oracle on the small workload (the size
+    at which an FP64 run is feasible on the host: 200 iterations), on identical
+    inputs: every iteration's patches, stratified start and occupancy cells /
+    jitter are drawn once and fed to the CPU oracle in float64
+    (oracle/train_oracle.py) and to the device iteration (graph mode, given
+    inputs) in each precision.  Returns each run's Chamfer / F-score (the
+    reference's evaluation, `accuracy`) and its ratio to the oracle's."""
+    from oracle.train_oracle import OracleTrainer
+    from supernormal_amd.runner import Runner, synthetic_conf
+    torch.set_num_threads(cpu_threads())
+    runs = {}
+    for p in precisions:
+        conf = synthetic_conf("sphere128", mlp_precision=p, step_mode="graph")
+        runs[p] = Runner(conf, fused=True, device=dev, seed=seed, rng_seed=seed)
+    r0 = runs[precisions[0]]
+    tr = OracleTrainer(r0.sdf_network, float(r0.deviation_network.variance.detach()),
+                       r0.occ_resolution, {"lr": r0.learning_rate, "loss_type": r0.loss_type},
+                       dtype=torch.float64)
+    ds = r0.dataset
+    g = torch.Generator(device=dev).manual_seed(seed + 99)
+    rng = np.random.default_rng(seed + 7)
+    E = r0.end_iter
+    t0 = time.perf_counter()
+    for i in range(E):
+        step = float(10 ** (np.log10(r0.start_step_size) - r0.slop_step * i))
+        if i % r0.occ_update_freq == 0:
+            idx, jit = tr.occupancy_cells(i)
+            tr.occupancy_update(i, n=r0.occ_update_freq, thre=r0.occ_threshold,
+                                cells=(idx, jit))
+            for r in runs.values():
+                if r.sf_step is not None:
+                    r.sf_step.drain()
+                r.renderer.occupancy_grid.update_cells(
+                    torch.from_numpy(idx), torch.from_numpy(jit), r.occ_eval_fn(),
+                    r.occ_threshold, 0.95)
+        if i % r0.increase_bindwidth_every == 0:
+            tr.field.bindwidth += 1
+            for r in runs.values():
+                r.sdf_network.increase_bandwidth()
+        o, d, pn, vi, nt, m = ds.gen_random_patches(r0.batch_size, generator=g)
+        near, far = ds.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+        u = torch.from_numpy(rng.random(near.shape[0], dtype=np.float32)).to(dev)
+        t_min = near + u * step
+        for r in runs.values():
+            r.iter_step = i
+            r.update_learning_rate()
+            r.sf_step.step(r.lr, step, use_graph=True,
+                           inputs=(o, d, pn, vi, nt, m, t_min, far))
+        cpu = [t.cpu() for t in (o, d, pn, vi, nt, m, near, far)]
+        tr.step(*cpu, step_size=step, lr=r0.lr, cap=r0.max_samples_per_ray, t_min=t_min.cpu())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {}
+    for p, r in runs.items():
+        out[p] = accuracy(r, resolution=resolution)
+    net = r0.sdf_network
+    with torch.no_grad():
+        net.encoding.params.copy_(tr.field.params.detach().float())
+        for l in range(net.num_layers - 1):
+            lin = getattr(net, f"lin{l}")
+            lin.weight_v.copy_(tr.field.v[l].detach().float())
+            lin.weight_g.copy_(tr.field.g[l].detach().float())
+            lin.bias.copy_(tr.field.b[l].detach().float())
+    net.bindwidth = tr.field.bindwidth
+    out["fp64_oracle"] = accuracy(r0, resolution=resolution)
+    ref = out["fp64_oracle"]["chamfer"]
+    for p in precisions:
+        out[p]["chamfer_ratio_to_fp64_oracle"] = round(out[p]["chamfer"] / ref, 4)
+        out[p]["within_2pct"] = bool(abs(out[p]["chamfer"] / ref - 1.0) <= 0.02)
+    out["workload"] = ("small workload (sphere r 0.5, 8 views 128^2, hash 8x2 T 2^15, MLP 32x1, "
+                       f"256 patches, occupancy 64^3, <= 32 samples/ray, {E} iterations), "
+                       f"identical inputs, mesh resolution {resolution}")
+    out["wall_s"] = round(wall, 1)
+    return out
 
 
 def plain_from_state(runner, iters):
@@ -410,6 +492,8 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-gate", action="store_true",
+                    help="skip the small workload's run against the FP64 oracle")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the timed run: 5 hash-grid backward calls on the last "
                          "iteration's points and 3 copies of 64 MiB (calibration), for "
@@ -499,7 +583,11 @@ def main(argv=None):
             out["plain_gpu"]["chamfer_ratio_device_to_plain"] = round(
                 out["accuracy"]["chamfer"] / pf["accuracy"]["chamfer"], 4)
     if not a.no_cpu and not a.no_cpu_small:
-        out["cpu_baseline"]["small_workload"] = cpu_small_workload(dev, a.seed)
+        out["cpu_baseline"]["small_workload"] = cpu_small_workload(a.seed)
+        log(f"CPU small workload: {out['cpu_baseline']['small_workload']['value']} it/s")
+    if not a.no_gate:
+        out["fp64_gate"] = fp64_gate(dev, a.seed)
+        log(f"FP64 gate: {json.dumps(out['fp64_gate'])}")
     out["bench_wall_s"] = round(time.perf_counter() - t_bench, 1)
     print(json.dumps(out), flush=True)
     return 0

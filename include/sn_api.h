@@ -58,6 +58,12 @@ int sn_counts_to_packed_info(const int32_t* counts, int32_t n, int32_t* packed_i
                              int32_t* total, void* workspace, size_t workspace_bytes,
                              void* stream);
 
+/* p[i] = value for i in [*lo_dev * lo_mult, min(*hi_dev * hi_mult, cap)):  */
+/* clears the part of a capacity-sized buffer that a sync-free iteration's   */
+/* device counts say is in use (a fill kernel: safe under graph capture).    */
+int sn_fill_span_dev(float* p, int64_t cap, const int32_t* lo_dev, int32_t lo_mult,
+                     const int32_t* hi_dev, int32_t hi_mult, float value, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Patch-based render weights: replaces                                      */
 /*   _C.weight_from_alpha_patch_based_forward_naive  (csrc/render_weight.cu:502-543)  */
@@ -308,6 +314,14 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
                   int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
                   float* g_h0, float* g_h0_dot, float* g_mlp, int32_t accumulate,
                   void* workspace, size_t workspace_bytes, void* stream);
+/* As sn_mlp_bwd_ad with only the encoding's columns of g_h0 / g_h0_dot,     */
+/* written as contiguous [n, enc_dims] rows g_enc / g_enc_dot (the layout the */
+/* hash-grid backward reads; positions are constants of the iteration).      */
+int sn_mlp_bwd_ad_enc(const float* x, const float* enc, const float* v, const float* enc_dot,
+                      int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                      int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                      float* g_enc, float* g_enc_dot, float* g_mlp, int32_t accumulate,
+                      void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused patch rendering + losses, forward and backward: replaces the torch  */
@@ -372,6 +386,41 @@ int sn_mt_emit(const float* values, int32_t rx, int32_t ry, int32_t rz, float is
                const float* bound_min, const float* bound_max, const int32_t* packed_info,
                float* triangles, void* stream);
 
+/* Marching cubes: replaces PyMCubes' marching_cubes(u, threshold) as called */
+/* by extract_geometry (models/renderer.py:27-34).  Corner case = sum of     */
+/* (value < iso) << corner in the usual corner numbering; one vertex per     */
+/* grid edge whose ends straddle iso, at the linear interpolation of the     */
+/* values along the edge, so the vertex set is the one any marching-cubes    */
+/* table gives; the triangulation of each case (faces with diagonal inside   */
+/* corners keep those corners apart, so the surface has no cracks) is        */
+/* derived by supernormal_amd/csrc/gen_mc_tables.py.  Triangles are wound so  */
+/* that their normal points towards value > iso.  sn_mc_count writes the     */
+/* triangle count of each of the (rx-1)(ry-1)(rz-1) cubes (x-major, as the   */
+/* values); scan it with sn_counts_to_packed_info; sn_mc_emit writes each    */
+/* cube's triangles at its base: 9 floats per triangle (world space, grid    */
+/* point (i,j,k) at bound_min + (i,j,k) * (bound_max-bound_min)/(r-1)) and,  */
+/* if vertex_ids is not NULL, 3 int64 per triangle: the vertex's edge id      */
+/* 3 * (lower grid point's linear index) + axis (equal ids = equal vertex).  */
+int sn_mc_count(const float* values, int32_t rx, int32_t ry, int32_t rz, float iso,
+                int32_t* counts, void* stream);
+int sn_mc_emit(const float* values, int32_t rx, int32_t ry, int32_t rz, float iso,
+               const float* bound_min, const float* bound_max, const int32_t* packed_info,
+               float* triangles, int64_t* vertex_ids, void* stream);
+
+/* First hit of each ray with a mesh from sn_mc_emit: replaces trimesh's     */
+/* mesh.ray.intersects_location(multiple_hits=False) in find_visible_points */
+/* (exp_runner.py:618-630).  The ray walks the grid's cubes in order from    */
+/* max(t_near, box entry) to min(t_far, box exit) (t_near / t_far nullable:  */
+/* 0 and +inf) and tests the triangles of each cube (packed_info from the    */
+/* scan of sn_mc_count).  t_hit = ray parameter of the nearest hit (+inf if  */
+/* none; rays_d need not be unit), tri_hit = its triangle or -1.  bound_min  */
+/* and bound_max are host pointers (3 floats), as in sn_mc_emit.             */
+int sn_mesh_first_hit(const float* rays_o, const float* rays_d, const float* t_near,
+                      const float* t_far, int32_t n_rays, int32_t rx, int32_t ry, int32_t rz,
+                      const float* bound_min, const float* bound_max,
+                      const int32_t* packed_info, const float* triangles, float* t_hit,
+                      int32_t* tri_hit, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Adam (torch.optim.Adam defaults, exp_runner.py:97, 205-207) over one flat */
 /* fp32 parameter buffer: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;        */
@@ -398,6 +447,17 @@ int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float
 /* (nullable) -- the count of iterations skipped for want of samples.        */
 int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_dev,
                     void* stream);
+
+/* Step guard (no counterpart in the reference, which has no NaN check):     */
+/* run = 1 if *n_samples > 0, the n_losses losses are finite and every one    */
+/* of the n gradients is finite, else 0; pass run to sn_step_advance and     */
+/* sn_adam_step_copy16 (run_dev) so that such an iteration leaves the        */
+/* parameters, moments and step count unchanged.  n_nonfinite (nullable,     */
+/* device int32) counts the iterations that had samples but a non-finite     */
+/* loss or gradient.  grads: 16-byte aligned.                                */
+int sn_step_guard(const int32_t* n_samples, const float* losses, int32_t n_losses,
+                  const float* grads, int64_t n, int32_t* run, int32_t* n_nonfinite,
+                  void* stream);
 /* dst[i] = (fp16) src[i] (initialises such a copy). */
 int sn_to_half(const float* src, void* dst, int64_t n, void* stream);
 
@@ -473,6 +533,18 @@ int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, con
                         int32_t base_resolution, float per_level_scale, int32_t n_active_levels,
                         const int32_t* n_active_dev,
                         const float* grad_enc, float* grad_params, float* grad_x, void* stream);
+/* The analytic x-gradient of the SDF given the MLP's d sdf / d h0 rows      */
+/* (sn_mlp_fwd's dsdf_dh0, h0 = [x, enc], row stride ld >= 3 + L*F):         */
+/* grad [n,3] = dsdf_dh0[:, :3] + sum over levels/corners of the table's     */
+/* contribution (fields.py:107-119: autograd.grad of the sdf w.r.t. x).      */
+/* Reads the fp32 parameters, or the fp16 copy params16 when not NULL.       */
+/* n_dev / n_active_dev: device counts as in sn_hashgrid_fwd_dev.            */
+int sn_hashgrid_dx_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                       const void* params16, int32_t n_levels, int32_t n_features,
+                       int32_t log2_hashmap_size, int32_t base_resolution,
+                       float per_level_scale, int32_t n_active_levels,
+                       const int32_t* n_active_dev, const float* dsdf_dh0, int32_t ld,
+                       float* grad, void* stream);
 /* Backward as sn_hashgrid_bwd_dev with a workspace (F <= 2, mode 2), no   */
 /* fp32 atomics: corner contributions (summed across the lanes of a         */
 /* wavefront whose points share a cell) become records in per-bucket lists  */

@@ -25,6 +25,7 @@ SIGNATURES = {
     "sn_pack_info_workspace_bytes": (_sz, [_i32]),
     "sn_pack_info": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _sz, _p]),
     "sn_scan_workspace_bytes": (_sz, [_i64]),
+    "sn_fill_span_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _i32, _f32, _p]),
     "sn_counts_to_packed_info": (C.c_int, [_p, _i32, _p, _p, _p, _sz, _p]),
     "sn_patch_weight_fwd": (C.c_int, [_p, _i32, _i32, _i64, _p, _p, _p, _p]),
     "sn_patch_weight_bwd": (C.c_int, [_p, _i32, _i32, _i64, _p, _p, _p, _p, _p]),
@@ -56,6 +57,8 @@ SIGNATURES = {
                              _i32, _p, _sz, _p]),
     "sn_mlp_bwd_ad": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p,
                                 _p, _p, _i32, _p, _sz, _p]),
+    "sn_mlp_bwd_ad_enc": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p,
+                                    _p, _p, _p, _i32, _p, _sz, _p]),
     "sn_render_step_waves": (C.c_int, [_i32, _i32]),
     "sn_render_step": (C.c_int, [_i32, _i32, _i32] + [_p] * 16 + [_f32, _f32, _f32, _i32]
                        + [_p] * 8),
@@ -79,10 +82,15 @@ SIGNATURES = {
     "sn_march_gather": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _p, _p, _p, _p]),
     "sn_mt_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
     "sn_mt_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p]),
+    "sn_mc_count": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p]),
+    "sn_mc_emit": (C.c_int, [_p, _i32, _i32, _i32, _f32, _p, _p, _p, _p, _p, _p]),
+    "sn_mesh_first_hit": (C.c_int, [_p, _p, _p, _p, _i32, _i32, _i32, _i32, _p, _p, _p, _p,
+                                    _p, _p, _p]),
     "sn_adam_step_copy16": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
                                       C.c_double, _i32, _p, _p, _i64, _i64, _p, _p]),
     "sn_step_advance": (C.c_int, [_p, _p, _p, _p]),
     "sn_to_half": (C.c_int, [_p, _p, _i64, _p]),
+    "sn_step_guard": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p]),
     "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
@@ -95,6 +103,8 @@ SIGNATURES = {
                                   _p, _p, _sz, _p]),
     "sn_hashgrid_fwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
     "sn_hashgrid_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p]),
+    "sn_hashgrid_dx_dev": (C.c_int, [_p, _i64, _p, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32,
+                                     _p, _p, _i32, _p, _p]),
     "sn_hashgrid_bwd_workspace_bytes": (_i64, [_i32, _i32, _i32, _i32, _f32, _i32]),
     "sn_hashgrid_bwd_ws_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _sz, _p]),
     "sn_hashgrid_bwd_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _p, _p]),

@@ -81,6 +81,44 @@ __global__ void step_advance_kernel(int32_t* __restrict__ step_dev,
   else if (skipped_dev) ++*skipped_dev;
 }
 
+// Step guard: run = 1 when the iteration has samples and its losses and
+// every gradient are finite, else 0 (the optimiser step is then skipped).
+__global__ void guard_init_kernel(const int32_t* __restrict__ n_samples,
+                                  const float* __restrict__ losses, int32_t n_losses,
+                                  int32_t* __restrict__ run) {
+  if (threadIdx.x != 0) return;
+  bool ok = *n_samples > 0;
+  for (int i = 0; i < n_losses; ++i) ok = ok && isfinite(losses[i]);
+  *run = ok ? 1 : 0;
+}
+
+__device__ __forceinline__ bool nonfinite_bits(float x) {
+  return (__float_as_uint(x) & 0x7f800000u) == 0x7f800000u;
+}
+
+__global__ void __launch_bounds__(256)
+guard_grads_kernel(const float* __restrict__ g, int64_t n, int32_t* __restrict__ run) {
+  if (*run == 0) return;  // uniform: nothing left to decide
+  const int64_t n4 = n / 4;
+  const float4* g4 = reinterpret_cast<const float4*>(g);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 v = g4[i];
+    bad |= nonfinite_bits(v.x) | nonfinite_bits(v.y) | nonfinite_bits(v.z) | nonfinite_bits(v.w);
+  }
+  for (int64_t i = n4 * 4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    bad |= nonfinite_bits(g[i]);
+  // every writer stores the same 0
+  if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) *run = 0;
+}
+
+__global__ void guard_count_kernel(const int32_t* __restrict__ n_samples,
+                                   const int32_t* __restrict__ run,
+                                   int32_t* __restrict__ n_nonfinite) {
+  if (threadIdx.x == 0 && *n_samples > 0 && *run == 0) ++*n_nonfinite;
+}
+
 __global__ void __launch_bounds__(256)
 to_half_kernel(const float* __restrict__ src, _Float16* __restrict__ dst, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -113,6 +151,22 @@ int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_
   SN_REQUIRE(step_dev && run_dev, "sn_step_advance: null pointer");
   step_advance_kernel<<<1, 64, 0, as_stream(stream)>>>(step_dev, run_dev, skipped_dev);
   return check_launch("sn_step_advance");
+}
+
+int sn_step_guard(const int32_t* n_samples, const float* losses, int32_t n_losses,
+                  const float* grads, int64_t n, int32_t* run, int32_t* n_nonfinite,
+                  void* stream) {
+  clear_error();
+  SN_REQUIRE(n_samples && run && n >= 0 && n_losses >= 0 && (n_losses == 0 || losses),
+             "sn_step_guard: bad arguments");
+  SN_REQUIRE(n == 0 || (grads && (uintptr_t)grads % 16 == 0),
+             "sn_step_guard: gradients must be a 16-byte aligned buffer");
+  hipStream_t s = as_stream(stream);
+  guard_init_kernel<<<1, 64, 0, s>>>(n_samples, losses, n_losses, run);
+  if (n > 0)
+    guard_grads_kernel<<<grid_for(ceil_div(n, 4), 256, 256 * 8), 256, 0, s>>>(grads, n, run);
+  if (n_nonfinite) guard_count_kernel<<<1, 64, 0, s>>>(n_samples, run, n_nonfinite);
+  return check_launch("sn_step_guard");
 }
 
 int sn_to_half(const float* src, void* dst, int64_t n, void* stream) {

@@ -126,6 +126,50 @@ __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restric
 // stride over the count read from the device.
 constexpr int kHashBlocks = 256 * 16;
 
+// The analytic x-gradient of the SDF (fields.py:107-119) from the MLP's
+// d sdf / d h0 rows (h0 = [x, enc], row stride ld): the encoding's part goes
+// through the table (a gather: sum over levels and corners of
+// scale_l * d w_c / d frac * (dsdf/denc_l . table row)), the position's part
+// is added last -- the order in which the backward to x (grad_x) plus the
+// direct term sums it.
+template <int F, typename TP>
+__global__ void __launch_bounds__(256)
+hash_dx_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+               const TP* __restrict__ params, HashGridDesc d, const float* __restrict__ dh0,
+               int32_t ld, float* __restrict__ grad) {
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int na = active_levels(d);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    const float* g = dh0 + i * (int64_t)ld;
+    float dx[3] = {0.f, 0.f, 0.f};
+    for (int l = 0; l < na; ++l) {
+      uint32_t idx[8];
+      float w[8], fr[3], dw[8][3];
+      hg_level(p, d, l, idx, w, fr);
+      hg_dweights(fr, dw);
+      const TP* tab = params + (int64_t)d.offset[l] * F;
+      float gl[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) gl[f] = g[3 + l * F + f];
+      float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        float sm = 0.f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) sm += gl[f] * (float)tab[(int64_t)idx[c] * F + f];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += dw[c][k] * sm;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) grad[3 * i + k] = dx[k] + g[k];
+  }
+}
+
 template <int F, typename TP = float, bool kPair = false>
 __global__ void __launch_bounds__(256)
 hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
@@ -1006,6 +1050,36 @@ int sn_hashgrid_fwd16_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                                                               0, as_stream(stream)>>>(
                                 x, n_cap, n_dev, p16, d, enc)));
   return check_launch("sn_hashgrid_fwd16");
+}
+
+int sn_hashgrid_dx_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,
+                       const void* params16, int32_t n_levels, int32_t n_features,
+                       int32_t log2_hashmap_size, int32_t base_resolution,
+                       float per_level_scale, int32_t n_active_levels,
+                       const int32_t* n_active_dev, const float* dsdf_dh0, int32_t ld,
+                       float* grad, void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_dx", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  d.n_active_dev = n_active_dev;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_dx: n < 0");
+  SN_REQUIRE(ld >= 3 + n_levels * n_features, "sn_hashgrid_dx: row stride %d < 3 + %d", ld,
+             n_levels * n_features);
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && (params || params16) && dsdf_dh0 && grad, "sn_hashgrid_dx: null pointer");
+  const unsigned blocks = grid_for(n_cap, 256, kHashBlocks);
+  hipStream_t s = as_stream(stream);
+  if (params16) {
+    const _Float16* p16 = reinterpret_cast<const _Float16*>(params16);
+    SN_DISPATCH_F(n_features, (hash_dx_kernel<kF, _Float16><<<blocks, 256, 0, s>>>(
+                                  x, n_cap, n_dev, p16, d, dsdf_dh0, ld, grad)));
+  } else {
+    SN_DISPATCH_F(n_features, (hash_dx_kernel<kF, float><<<blocks, 256, 0, s>>>(
+                                  x, n_cap, n_dev, params, d, dsdf_dh0, ld, grad)));
+  }
+  return check_launch("sn_hashgrid_dx");
 }
 
 int sn_hashgrid_fwd(const float* x, int64_t n, const float* params, int32_t n_levels,

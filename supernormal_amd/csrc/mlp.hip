@@ -475,7 +475,8 @@ mlp_bwd_ad_kernel(const float* __restrict__ x, const float* __restrict__ enc, co
                   const float* __restrict__ encd, int E, int64_t n_cap,
                   const int32_t* __restrict__ n_dev, const float* __restrict__ mlp, float beta,
                   const float* __restrict__ a_sdf, float* __restrict__ gb_h0,
-                  float* __restrict__ gt_h0, float* __restrict__ partials) {
+                  float* __restrict__ gt_h0, float* __restrict__ partials, int out_ld,
+                  int out_c0) {
   using M = Mlp<H, NH, DP>;
   extern __shared__ float smem[];
   float* sw = smem;
@@ -593,9 +594,9 @@ mlp_bwd_ad_kernel(const float* __restrict__ x, const float* __restrict__ enc, co
         for (int e = 0; e < 4; ++e) {
           const int64_t row = row0 + rb * 16 + 4 * q + e;
           const int c = ib * 16 + r;
-          if (row < n && c < d) {
-            if (gb_h0) gb_h0[row * d + c] = o1[rb][ib][e];
-            if (gt_h0) gt_h0[row * d + c] = o2[rb][ib][e];
+          if (row < n && c < d && c >= out_c0) {
+            if (gb_h0) gb_h0[row * out_ld + (c - out_c0)] = o1[rb][ib][e];
+            if (gt_h0) gt_h0[row * out_ld + (c - out_c0)] = o2[rb][ib][e];
           }
         }
     __syncthreads();
@@ -772,12 +773,12 @@ int sn_mlp_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap
   return check_launch("sn_mlp_bwd");
 }
 
-int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float* enc_dot,
-                  int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
-                  int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
-                  float* g_h0, float* g_h0_dot, float* g_mlp, int32_t accumulate,
-                  void* workspace, size_t workspace_bytes, void* stream) {
-  clear_error();
+static int mlp_bwd_ad(const float* x, const float* enc, const float* v, const float* enc_dot,
+                      int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                      int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                      float* g_h0, float* g_h0_dot, float* g_mlp, int32_t accumulate,
+                      void* workspace, size_t workspace_bytes, void* stream, int out_ld,
+                      int out_c0) {
   MlpCfg c;
   int rc = mlp_cfg("sn_mlp_bwd_ad", enc_dims, d_hidden, n_layers, &c);
   if (rc) return rc;
@@ -797,10 +798,33 @@ int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float*
     const size_t sm = smem_bytes<kH, kNH, kDP>(T, 2);
     allow_smem(mlp_bwd_ad_kernel<kH, kNH, kDP, T>, sm);
     mlp_bwd_ad_kernel<kH, kNH, kDP, T><<<grid, 64 * kWaves, sm, s>>>(
-        x, enc, v, enc_dot, enc_dims, n_cap, n_dev, mlp, beta, g_sdf, g_h0, g_h0_dot, part);
+        x, enc, v, enc_dot, enc_dims, n_cap, n_dev, mlp, beta, g_sdf, g_h0, g_h0_dot, part,
+        out_ld, out_c0);
   });
   if ((rc = mlp_reduce_rows(part, (int64_t)grid * kWaves, np, g_mlp, accumulate, s))) return rc;
   return check_launch("sn_mlp_bwd_ad");
+}
+
+int sn_mlp_bwd_ad(const float* x, const float* enc, const float* v, const float* enc_dot,
+                  int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                  int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                  float* g_h0, float* g_h0_dot, float* g_mlp, int32_t accumulate,
+                  void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  return mlp_bwd_ad(x, enc, v, enc_dot, enc_dims, n_cap, n_dev, mlp, d_hidden, n_layers, beta,
+                    g_sdf, g_h0, g_h0_dot, g_mlp, accumulate, workspace, workspace_bytes, stream,
+                    3 + enc_dims, 0);
+}
+
+int sn_mlp_bwd_ad_enc(const float* x, const float* enc, const float* v, const float* enc_dot,
+                      int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                      int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                      float* g_enc, float* g_enc_dot, float* g_mlp, int32_t accumulate,
+                      void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  return mlp_bwd_ad(x, enc, v, enc_dot, enc_dims, n_cap, n_dev, mlp, d_hidden, n_layers, beta,
+                    g_sdf, g_enc, g_enc_dot, g_mlp, accumulate, workspace, workspace_bytes,
+                    stream, enc_dims, 3);
 }
 
 }  // extern "C"

@@ -43,6 +43,19 @@ int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
   return check_launch("fill_i32");
 }
 
+// p[i] = v for i in [lo * lo_mult, min(hi * hi_mult, cap)) with lo / hi read
+// from the device (counts of a sync-free iteration).
+__global__ void fill_span_kernel(float* __restrict__ p, int64_t cap,
+                                 const int32_t* __restrict__ lo_dev, int32_t lo_mult,
+                                 const int32_t* __restrict__ hi_dev, int32_t hi_mult, float v) {
+  const int64_t lo = (int64_t)max(*lo_dev, 0) * lo_mult;
+  int64_t hi = (int64_t)max(*hi_dev, 0) * hi_mult;
+  hi = hi < cap ? hi : cap;
+  for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 int fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   if (n <= 0) return SN_OK;
   fill_f32_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
@@ -268,6 +281,17 @@ using namespace sn;
 extern "C" {
 
 size_t sn_scan_workspace_bytes(int64_t n) { return scan_ws_bytes(n); }
+
+int sn_fill_span_dev(float* p, int64_t cap, const int32_t* lo_dev, int32_t lo_mult,
+                     const int32_t* hi_dev, int32_t hi_mult, float value, void* stream) {
+  clear_error();
+  SN_REQUIRE(cap >= 0 && lo_mult >= 0 && hi_mult >= 0, "sn_fill_span_dev: negative size");
+  if (cap == 0) return SN_OK;
+  SN_REQUIRE(p && lo_dev && hi_dev, "sn_fill_span_dev: null pointer");
+  fill_span_kernel<<<grid_for(cap, 256), 256, 0, as_stream(stream)>>>(p, cap, lo_dev, lo_mult,
+                                                                      hi_dev, hi_mult, value);
+  return check_launch("sn_fill_span_dev");
+}
 
 int sn_counts_to_packed_info(const int32_t* counts, int32_t n, int32_t* packed_info,
                              int32_t* total, void* ws, size_t ws_bytes, void* stream) {

@@ -140,6 +140,8 @@ class FusedStep:
             float(mask_weight)
         self.l1 = loss_type == "l1"
         self.early_stop_eps = early_stop_eps
+        # the visibility decision of the last call's marched samples (bool [S0])
+        self.last_keep = None
 
     def inv_s(self):
         return torch.exp(self.dev_net.variance * 10.0).clip(1e-6, 1e6).reshape(1)
@@ -185,6 +187,7 @@ class FusedStep:
             alpha = ((c - e + 1e-5) / (c + 1e-5)).clip(0.0, 1.0)
             keep = render_visibility(alpha[:, None], packed_info=packed0,
                                      early_stop_eps=self.early_stop_eps, alpha_thre=0.0)
+            self.last_keep = keep
             ri, t0, t1 = ri0[keep], t00[keep], t10[keep]
         S = ri.numel()
         if S == 0:

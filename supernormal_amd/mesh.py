@@ -70,25 +70,10 @@ def first_hits(sdf_fn, rays_o, rays_d, near, far, step=2e-3, n_bisect=10, chunk=
 @torch.no_grad()
 def visible_surface_points(sdf_fn, dataset, rays_per_view=20000, seed=0, step=2e-3):
     """First hits of the learned SDF along masked training-pixel rays (the
-    rays of all views marched together)."""
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    rays = []
-    for i in dataset.train_images:
-        m = dataset.masks[i].reshape(-1).nonzero()[:, 0].cpu()
-        if m.numel() == 0:
-            continue
-        pick = m[torch.randint(0, m.numel(), (min(rays_per_view, m.numel()),), generator=g)]
-        ys, xs = (pick // dataset.W).to(dataset.device), (pick % dataset.W).to(dataset.device)
-        p = torch.stack([xs.float(), ys.float(), torch.ones_like(xs, dtype=torch.float32)], -1)
-        p = p @ dataset.intrinsics_all_inv[i, :3, :3].T
-        d = p / p.norm(dim=-1, keepdim=True)
-        d = d @ dataset.pose_all[i, :3, :3].T
-        o = dataset.pose_all[i, :3, 3].expand(d.shape).contiguous()
-        rays.append((o, d))
-    if not rays:
+    rays of all views marched together), without building a mesh."""
+    o, d = _pixel_rays(dataset, rays_per_view, seed)
+    if o is None:
         return np.zeros((0, 3), np.float32)
-    o = torch.cat([r[0] for r in rays])
-    d = torch.cat([r[1] for r in rays])
     near, far = dataset.near_far_from_sphere(o, d)
     pts, _ = first_hits(sdf_fn, o, d, near, far, step=step)
     return pts.cpu().numpy()
@@ -187,19 +172,137 @@ def marching_tetrahedra_device(w, iso, bound_min, bound_max):
     return tris[:t]
 
 
+class DeviceMesh:
+    """A marching-cubes mesh kept on the device with the grid it came from:
+    triangles [T,3,3] (world space), packed_info [(rx-1)(ry-1)(rz-1), 2] of
+    each cube's triangles (base, count), vertex edge ids [T,3] (or None), the
+    grid's shape and bounds."""
+
+    def __init__(self, tris, packed, shape, bound_min, bound_max, vertex_ids=None):
+        self.tris, self.packed, self.shape = tris, packed, tuple(int(v) for v in shape)
+        self.bound_min = [float(v) for v in bound_min]
+        self.bound_max = [float(v) for v in bound_max]
+        self.vertex_ids = vertex_ids
+
+    def indexed(self):
+        """(vertices [V,3] float64, triangles [T,3] int64) with one vertex per
+        grid edge, as PyMCubes returns them (numpy, host)."""
+        if self.vertex_ids is None:
+            raise ValueError("mesh built without vertex ids")
+        ids = self.vertex_ids.reshape(-1)
+        uniq, inv = torch.unique(ids, return_inverse=True)
+        first = torch.full((uniq.numel(),), ids.numel(), dtype=torch.int64, device=ids.device)
+        first.scatter_reduce_(0, inv, torch.arange(ids.numel(), device=ids.device), "amin")
+        v = self.tris.reshape(-1, 3)[first]
+        return v.cpu().numpy().astype(np.float64), inv.reshape(-1, 3).cpu().numpy()
+
+
+def marching_cubes_device(w, iso, bound_min, bound_max, vertex_ids=True):
+    """Marching cubes of {w = iso} on the device (sn_mc_count / scan /
+    sn_mc_emit; PyMCubes' mesher in renderer.py:27-34): a DeviceMesh whose
+    triangles are wound towards w > iso."""
+    from . import _lib
+    from ._lib import call, ptr, stream_of, workspace
+    w = w.contiguous().to(torch.float32)
+    rx, ry, rz = (int(v) for v in w.shape)
+    dev = w.device
+    s = stream_of(dev)
+    n = (rx - 1) * (ry - 1) * (rz - 1)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    call("sn_mc_count", ptr(w), rx, ry, rz, float(iso), ptr(counts), s)
+    packed = torch.empty((n, 2), dtype=torch.int32, device=dev)
+    total = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = workspace(_lib.lib().sn_scan_workspace_bytes(n), dev)
+    call("sn_counts_to_packed_info", ptr(counts), n, ptr(packed), ptr(total), ptr(ws),
+         ws.numel(), s)
+    t = int(total.item())
+    tris = torch.empty((max(t, 1), 3, 3), dtype=torch.float32, device=dev)
+    ids = torch.empty((max(t, 1), 3), dtype=torch.int64, device=dev) if vertex_ids else None
+    if t:
+        call("sn_mc_emit", ptr(w), rx, ry, rz, float(iso), ctypes_floats(bound_min),
+             ctypes_floats(bound_max), ptr(packed), ptr(tris), ptr(ids), s)
+    return DeviceMesh(tris[:t], packed, (rx, ry, rz), bound_min, bound_max,
+                      ids[:t] if ids is not None else None)
+
+
+@torch.no_grad()
+def mesh_first_hits(mesh, rays_o, rays_d, near=None, far=None):
+    """(t [N], triangle [N]) of each ray's first hit with a DeviceMesh
+    (sn_mesh_first_hit; trimesh's intersects_location with multiple_hits=False
+    in exp_runner.py:618-630); t = inf and triangle = -1 for a miss."""
+    from ._lib import call, ptr, stream_of
+    o = rays_o.contiguous().to(torch.float32)
+    d = rays_d.contiguous().to(torch.float32)
+    n = o.shape[0]
+    dev = o.device
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    tri = torch.empty(n, dtype=torch.int32, device=dev)
+    nr = near.contiguous().to(torch.float32) if near is not None else None
+    fr = far.contiguous().to(torch.float32) if far is not None else None
+    rx, ry, rz = mesh.shape
+    call("sn_mesh_first_hit", ptr(o), ptr(d), ptr(nr), ptr(fr), n, rx, ry, rz,
+         ctypes_floats(mesh.bound_min), ctypes_floats(mesh.bound_max), ptr(mesh.packed),
+         ptr(mesh.tris), ptr(t), ptr(tri), stream_of(dev))
+    return t, tri
+
+
+def _pixel_rays(dataset, rays_per_view, seed):
+    """Unit-direction rays of masked training pixels, all views together
+    (gen_rays_at(within_mask=True), exp_runner.py:621-623); rays_per_view
+    None = every masked pixel."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    rays = []
+    for i in dataset.train_images:
+        m = dataset.masks[i].reshape(-1).nonzero()[:, 0].cpu()
+        if m.numel() == 0:
+            continue
+        if rays_per_view is not None:
+            m = m[torch.randint(0, m.numel(), (min(rays_per_view, m.numel()),), generator=g)]
+        ys, xs = (m // dataset.W).to(dataset.device), (m % dataset.W).to(dataset.device)
+        p = torch.stack([xs.float(), ys.float(), torch.ones_like(xs, dtype=torch.float32)], -1)
+        p = p @ dataset.intrinsics_all_inv[i, :3, :3].T
+        d = p / p.norm(dim=-1, keepdim=True)
+        d = d @ dataset.pose_all[i, :3, :3].T
+        o = dataset.pose_all[i, :3, 3].expand(d.shape).contiguous()
+        rays.append((o, d))
+    if not rays:
+        return None, None
+    return torch.cat([r[0] for r in rays]), torch.cat([r[1] for r in rays])
+
+
+@torch.no_grad()
+def visible_mesh_points(mesh, dataset, rays_per_view=None, seed=0):
+    """find_visible_points (exp_runner.py:618-630): first hits of masked
+    training-pixel rays with the extracted mesh; numpy [M,3]."""
+    o, d = _pixel_rays(dataset, rays_per_view, seed)
+    if o is None:
+        return np.zeros((0, 3), np.float32)
+    d = d / d.norm(dim=-1, keepdim=True)
+    t, tri = mesh_first_hits(mesh, o, d)
+    hit = tri >= 0
+    return (o[hit] + d[hit] * t[hit, None]).cpu().numpy()
+
+
 def ctypes_floats(v):
     import ctypes
     return (ctypes.c_float * 3)(*[float(x) for x in v])
 
 
-def extract_geometry(bound_min, bound_max, resolution, threshold, query_func, device=None):
-    """(vertices [3T, 3] in world units, triangles [T, 3]) of {query_func =
+def extract_geometry(bound_min, bound_max, resolution, threshold, query_func, device=None,
+                     mesher="mc"):
+    """(vertices [V, 3] in world units, triangles [T, 3]) of {query_func =
     threshold} (renderer.py:27-34, where query_func is -sdf and the mesher is
-    PyMCubes' marching cubes; here marching tetrahedra on the GPU, the same
-    surface up to the triangulation inside each cube).  Triangles are oriented
-    towards query_func < threshold (outside for query_func = -sdf)."""
+    PyMCubes' marching cubes).  mesher "mc": marching cubes on the GPU, one
+    vertex per grid edge crossing the level (PyMCubes' vertex set);
+    "mt": marching tetrahedra (a triangle soup, 3 vertices per triangle).
+    Triangles are oriented towards query_func < threshold (outside for
+    query_func = -sdf)."""
     device = device or torch.device("cuda")
     u = extract_fields_device(bound_min, bound_max, resolution, query_func, device)
+    if mesher == "mc":
+        return marching_cubes_device(-u, -float(threshold), bound_min, bound_max).indexed()
+    if mesher != "mt":
+        raise ValueError(f"unknown mesher {mesher!r}")
     tris = marching_tetrahedra_device(-u, -float(threshold), bound_min, bound_max)
     v = tris.reshape(-1, 3).cpu().numpy().astype(np.float64)
     f = np.arange(v.shape[0], dtype=np.int64).reshape(-1, 3)

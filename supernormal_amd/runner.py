@@ -144,7 +144,7 @@ class Runner:
             self.optimizer, self.dataset, self.batch_size, self.patch_size,
             self.max_samples_per_ray, self.normal_weight, self.eikonal_weight,
             self.mask_weight, self.loss_type, precision=self.mlp_precision,
-            device=self.device, seed=self.rng_seed)
+            device=self.device, seed=self.rng_seed, method=self.gradient_method)
 
     # ------------------------------------------------------------------ train
     def update_learning_rate(self):
@@ -241,10 +241,16 @@ class Runner:
             self.eikonal_weight * eik
 
     def train(self, n_iters: Optional[int] = None, log=print):
+        """The loop of exp_runner.py:145-220 for the remaining iterations (or
+        n_iters).  The schedules (step size, occupancy updates) follow the
+        absolute iteration, so a resumed run continues them; the reference
+        restarts them at 0 on resume (its loop index, exp_runner.py:146-155)."""
         self.update_learning_rate()
         res_step = (self.end_iter - self.iter_step) if n_iters is None else n_iters
+        first = self.iter_step
         t0 = time.time()
-        for iter_i in range(res_step):
+        for k in range(res_step):
+            iter_i = first + k
             loss = self.train_iteration(iter_i)
             if loss is not None and self.iter_step % self.report_freq == 0:
                 lv = float(loss)
@@ -277,7 +283,13 @@ class Runner:
             "occupancy_grid": self.renderer.occupancy_grid.state_dict(),
             "rng_cpu": torch.get_rng_state(),
             "rng_cuda": torch.cuda.get_rng_state(self.device) if self.device.type == "cuda" else None,
+            "rng_sampler": self.sf_step.sampler_rng_state() if self.sf_step is not None else None,
+            "rng_occupancy": (self.renderer.occupancy_grid.generator.get_state()
+                              if self.renderer.occupancy_grid.generator is not None else None),
         }
+        if self.sf_step is not None and self.sf_step.gen is None:
+            # the sampler draws from torch's default generator
+            ck["rng_cuda"] = ck["rng_sampler"]
         torch.save(ck, self._ckpt_path())
 
     def load_checkpoint(self, path=None):
@@ -286,7 +298,9 @@ class Runner:
             names = sorted(n for n in os.listdir(d) if n.endswith(".pth")
                            and int(n[5:-4]) <= self.end_iter)
             path = os.path.join(d, names[-1])
-        ck = torch.load(path, map_location=self.device, weights_only=True)
+        # on the host: the RNG states must stay CPU byte tensors; the rest is
+        # copied into the device buffers below
+        ck = torch.load(path, map_location="cpu", weights_only=True)
         self.sdf_network.load_state_dict(ck["sdf_network_fine"])
         self.deviation_network.load_state_dict(ck["variance_network_fine"])
         # parameters were re-pointed at the flat buffer; copy values back into it
@@ -301,11 +315,15 @@ class Runner:
         torch.set_rng_state(ck["rng_cpu"])
         if ck.get("rng_cuda") is not None:
             torch.cuda.set_rng_state(ck["rng_cuda"], self.device)
+        if ck.get("rng_occupancy") is not None and self.renderer.occupancy_grid.generator is not None:
+            self.renderer.occupancy_grid.generator.set_state(ck["rng_occupancy"])
         if self.sf_step is not None:
             # new parameter buffers: new step buffers and graph
             self.sf_step.drain()
             torch.cuda.synchronize(self.device)
             self._make_sync_free_step()
+            if ck.get("rng_sampler") is not None:
+                self.sf_step.set_sampler_rng_state(ck["rng_sampler"])
 
 
 def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=None, lr=None,
@@ -315,7 +333,7 @@ def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=None, lr=N
 
     sphere128: sphere r 0.5, 8 views 128^2, hash 8 levels x 2, T 2^15, base 16,
                x1.5, MLP 32 x 1 hidden, 256 patches, occupancy 64^3, <= 32
-               samples/ray, lr 1e-2, 200 iterations.
+               samples/ray, lr 1e-2, 200 iterations, step size 1e-2.
     tbs612:    torus + box + 3 spheres, 20 views 612x512, hash 16 x 2, T 2^19,
                base 16, x1.38, MLP 64 x 2 hidden, 2048 patches (18 432 rays),
                occupancy 128^3, bandwidth +1 / 250 its, sample capacity 1024
@@ -357,7 +375,13 @@ def synthetic_conf(workload="tbs612", gradient_method="dfd", end_iter=None, lr=N
         train["batch_size"] = batch_size
     if max_samples_per_ray is not None:
         rm["max_samples_per_ray"] = max_samples_per_ray
-    rm.update(start_step_size=1e-2, end_step_size=1e-3, occ_threshold=0.1, occ_update_freq=8)
+    # the reference's step-size decay 1e-2 -> 1e-3; the small workload keeps 1e-2:
+    # its 32-sample cap spans only 0.032 of a ray at 1e-3, less than the occupied
+    # shell in front of the surface, and training with the decay diverges (the
+    # CPU oracle's Chamfer over 200 iterations: 0.053 -> 0.41 with the decay,
+    # 0.014 -> 0.0098 with 1e-2)
+    rm.update(start_step_size=1e-2, end_step_size=1e-2 if workload == "sphere128" else 1e-3,
+              occ_threshold=0.1, occ_update_freq=8)
     return Conf(general=Conf(base_exp_dir=f"./exp/{workload}"),
                 dataset=Conf(synthetic=workload),
                 train=Conf(train), val=Conf(report_freq=100, save_freq=10 ** 9),

@@ -75,7 +75,7 @@ class SyncFreeStep:
     def __init__(self, sdf_network, deviation_network, occupancy_grid, flat, dataset,
                  batch_size, patch_size=3, max_samples_per_ray=128, normal_weight=1.0,
                  eikonal_weight=1.0, mask_weight=1.0, loss_type="l2", early_stop_eps=1e-8,
-                 precision="fp16", device="cuda", seed=None, half_table=None):
+                 precision="fp16", device="cuda", seed=None, half_table=None, method="dfd"):
         if max_samples_per_ray <= 0:
             raise ValueError("SyncFreeStep needs max_samples_per_ray > 0 (it bounds the buffers)")
         if precision not in ("fp32", "fp16"):
@@ -92,6 +92,12 @@ class SyncFreeStep:
         self.l1 = loss_type == "l1"
         self.eps = float(early_stop_eps)
         self.precision = precision
+        if method not in ("dfd", "ad"):
+            raise ValueError("gradient method must be 'dfd' or 'ad'")
+        # 'ad': the rendered normal is the analytic x-gradient of the SDF
+        # (renderer.py:225-226), its loss differentiated twice; the patch points'
+        # MLP runs on the fp32 kernels (the forward-over-reverse backward is fp32)
+        self.ad = method == "ad"
         # forward gathers from an fp16 copy of the hash table, refreshed by each
         # Adam step (fp32 master table and gradients); default with precision fp16
         self.half_table = (precision == "fp16") if half_table is None else bool(half_table)
@@ -108,6 +114,7 @@ class SyncFreeStep:
         self.graph_given = None  # the march alone, for iterations on given inputs
         # the next iteration's march when it ran ahead: (event, step size)
         self._pending = None
+        self._rng_pending = None
         self.side = None
         # optional timing of the hash-grid backward (events between replays)
         self.time_hash_bwd = False
@@ -155,6 +162,10 @@ class SyncFreeStep:
         self.stat = torch.zeros(8, **i32)
         self.n0, self.ns, self.ne0, self.ne = (self.stat[k:k + 1] for k in range(4))
         self.n_capped, self.n_skipped = self.stat[4:5], self.stat[5:6]
+        # whether this iteration's optimiser step runs (samples, finite losses
+        # and gradients), and a running count of iterations skipped for a
+        # non-finite loss or gradient
+        self.run, self.n_nonfinite = self.stat[6:7], self.stat[7:8]
         # the four per-iteration counts summed over iterations (int64)
         self.stat_acc = torch.zeros(4, dtype=torch.int64, device=dev)
         # marching
@@ -196,6 +207,14 @@ class SyncFreeStep:
         self.sdf = torch.zeros(self.c1, **f32)
         self.g_sdf = torch.zeros(self.c1, **f32)
         self.g_enc = torch.zeros(self.c1, E, **f32)
+        if self.ad:
+            # d sdf / d h0 rows, the analytic gradient of every patch point, its
+            # cotangent from the render step, and the tangents of the backward
+            self.dh0 = torch.zeros(self.c1, 3 + E, **f32)
+            self.grad_pts = torch.zeros(self.c1, 3, **f32)
+            self.g_grad = torch.zeros(self.c1, 3, **f32)
+            self.enc_dot = torch.zeros(self.c1, E, **f32)
+            self.g_enc_dot = torch.zeros(self.c1, E, **f32)
         # render step
         self.losses = torch.zeros(4, **f32)
         waves = lib.sn_render_step_waves(B, P)
@@ -208,7 +227,9 @@ class SyncFreeStep:
         hb = bucket_workspace_bytes(self.cfg.hash_args(self.cfg.grid["n_levels"]), self.c1)
         self.ws_hash = nb(hb) if hb else None
         cfg = self.cfg
-        if self.precision == "fp16":
+        if self.ad:
+            wsb = lib.sn_mlp_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers, 1)
+        elif self.precision == "fp16":
             wsb = lib.sn_mlp16_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers)
         else:
             wsb = lib.sn_mlp_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers, 0)
@@ -297,6 +318,15 @@ class SyncFreeStep:
         self.t_min.copy_(near)
         self.t_max.copy_(far)
 
+    def _load_given(self, inputs):
+        """Given inputs (rays_o, rays_d, plane_n, v_inv, n_true, mask[, t_min,
+        t_max]) into the static buffers; without t_min / t_max the centre rays
+        start at near on the unit sphere, unjittered."""
+        if len(inputs) not in (6, 8):
+            raise RuntimeError(f"given inputs: 6 or 8 tensors, got {len(inputs)}")
+        t_min, t_max = (inputs[6], inputs[7]) if len(inputs) == 8 else (None, None)
+        self._load(*inputs[:6], stratified=False, t_min=t_min, t_max=t_max)
+
     def _hash_args(self):
         """Hash-grid arguments with every level as the bound; the active count
         is read from self.levels_dev (set by step()), so a bandwidth increase
@@ -340,7 +370,7 @@ class SyncFreeStep:
         if inputs is None:
             self._sample()
         elif not isinstance(inputs, str):  # "keep": the buffers hold the inputs already
-            self._load(*inputs, stratified=False)
+            self._load_given(inputs)
         # one march into per-ray slots, then packed (= count + fill of the reference);
         # (the byte grid: brick masks, sn_ray_march_padded_bricks, measured no faster
         # with the wavefront-per-ray march)
@@ -384,15 +414,22 @@ class SyncFreeStep:
              ptr(self.rays_o), ptr(self.rays_d), ptr(self.plane_n), self.Ph, self.Pw,
              ptr(self.didx), ptr(self.nd), ptr(self.pts), ptr(self.ne), ptr(self.ws_step),
              self.ws_step.numel(), s)
-        self._sdf_fwd(self.pts, self.enc, self.sdf, c1, self.ne, mlp_d, s)
+        if self.ad:
+            self._sdf_and_gradient(c1, mlp_d, s)
+        else:
+            self._sdf_fwd(self.pts, self.enc, self.sdf, c1, self.ne, mlp_d, s)
+        ad = self.ad
         call("sn_render_step", B, self.Ph, self.Pw, ptr(self.rays_o), ptr(self.rays_d),
              ptr(self.plane_n), ptr(self.v_inv), ptr(self.n_true), ptr(self.mask),
              ptr(self.packed), ptr(self.t0), ptr(self.t1), ptr(self.didx), ptr(self.sdf), None,
-             None, ptr(self.inv_s), ptr(self.mask_sum), ptr(self.ns), self.normal_w, self.eik_w,
-             self.mask_w, 1 if self.l1 else 0, ptr(self.g_sdf), None, None, ptr(self.losses),
+             ptr(self.grad_pts) if ad else None, ptr(self.inv_s), ptr(self.mask_sum),
+             ptr(self.ns), self.normal_w, self.eik_w, self.mask_w, 1 if self.l1 else 0,
+             ptr(self.g_sdf), None, ptr(self.g_grad) if ad else None, ptr(self.losses),
              ptr(self.partials), None, None, s)
         # backward: MLP, hash grid (atomics straight into the flat gradient)
-        if self.precision == "fp16":
+        if ad:
+            self._ad_backward(c1, mlp_d, s)
+        elif self.precision == "fp16":
             call("sn_mlp16_bwd", ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne),
                  ptr(mlp_d), cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.g_sdf),
                  ptr(self.g_enc), None, ptr(self.g_mlp), 0, ptr(self.ws_mlp),
@@ -410,12 +447,53 @@ class SyncFreeStep:
         # this part is done, and it rewrites n0)
         self.stat_acc.add_(self.stat[:4])
 
+    def _sdf_and_gradient(self, c1, mlp_d, s):
+        """'ad': sdf and the analytic x-gradient of every patch point
+        (fields.py:107-119): MLP forward with d sdf / d h0, then the table's part
+        of the gradient by a gather."""
+        cfg = self.cfg
+        params = self.net.encoding.params
+        if self.table16 is not None:
+            call("sn_hashgrid_fwd16_dev", ptr(self.pts), c1, ptr(self.ne), ptr(self.table16),
+                 *self._hash_args(), ptr(self.levels_dev), ptr(self.enc), s)
+        else:
+            call("sn_hashgrid_fwd_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
+                 *self._hash_args(), ptr(self.levels_dev), ptr(self.enc), s)
+        call("sn_mlp_fwd", ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne), ptr(mlp_d),
+             cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.sdf), ptr(self.dh0), s)
+        call("sn_hashgrid_dx_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
+             ptr(self.table16), *self._hash_args(), ptr(self.levels_dev), ptr(self.dh0),
+             3 + cfg.E, ptr(self.grad_pts), s)
+
+    def _ad_backward(self, c1, mlp_d, s):
+        """'ad': the double backward through the analytic gradient (the
+        reference's autograd.grad(create_graph=True)): the gradient's cotangent
+        v (zero on the samples' end points, which carry no normal), the
+        encoding's tangent along v, the forward-over-reverse MLP backward."""
+        cfg = self.cfg
+        params = self.net.encoding.params
+        call("sn_fill_span_dev", ptr(self.g_grad), c1 * 3, ptr(self.ns), self.P * 3,
+             ptr(self.ne), 3, 0.0, s)
+        call("sn_hashgrid_bwd_bwd_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
+             *self._hash_args(), ptr(self.levels_dev), None, ptr(self.g_grad),
+             ptr(self.enc_dot), None, None, s)
+        call("sn_mlp_bwd_ad_enc", ptr(self.pts), ptr(self.enc), ptr(self.g_grad),
+             ptr(self.enc_dot), cfg.E, c1, ptr(self.ne), ptr(mlp_d), cfg.d_hidden,
+             cfg.n_layers, cfg.beta, ptr(self.g_sdf), ptr(self.g_enc), ptr(self.g_enc_dot),
+             ptr(self.g_mlp), 0, ptr(self.ws_mlp), self.ws_mlp.numel(), s)
+
     def _part_hash(self):
         """Hash-grid backward into the flat gradient buffer (bucket records,
-        then per-bucket sums)."""
-        hashgrid_bwd(self.pts, self.c1, self.ne, self.net.encoding.params, self._hash_args(),
-                     self.g_enc, self.net.encoding.params.grad, None, stream_of(self.dev),
-                     ws=self.ws_hash, n_active_dev=self.levels_dev)
+        then per-bucket sums); with 'ad' also the table's part of the double
+        backward (fp32 atomics)."""
+        params = self.net.encoding.params
+        s = stream_of(self.dev)
+        hashgrid_bwd(self.pts, self.c1, self.ne, params, self._hash_args(), self.g_enc,
+                     params.grad, None, s, ws=self.ws_hash, n_active_dev=self.levels_dev)
+        if self.ad:
+            call("sn_hashgrid_bwd_bwd_dev", ptr(self.pts), self.c1, ptr(self.ne), ptr(params),
+                 *self._hash_args(), ptr(self.levels_dev), ptr(self.g_enc_dot),
+                 ptr(self.g_grad), None, ptr(params.grad), None, s)
 
     def _part_adam(self):
         """Adam with the learning rate and step count on the device."""
@@ -423,13 +501,16 @@ class SyncFreeStep:
         # an iteration with no visible sample is skipped (parameters, moments and
         # step count unchanged; the reference raises KeyError there,
         # exp_runner.py:182) and counted in n_skipped
-        call("sn_step_advance", ptr(self.adam_t), ptr(self.ns), ptr(self.n_skipped), s)
         f = self.flat
+        # ... and so is one whose losses or gradients are not finite
+        call("sn_step_guard", ptr(self.ns), ptr(self.losses), 3, ptr(f.grad), f.n,
+             ptr(self.run), ptr(self.n_nonfinite), s)
+        call("sn_step_advance", ptr(self.adam_t), ptr(self.run), ptr(self.n_skipped), s)
         t16 = self.table16
         call("sn_adam_step_copy16", ptr(f.data), ptr(f.grad), ptr(f.exp_avg),
              ptr(f.exp_avg_sq), f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0,
              ptr(self.adam_t), ptr(t16), self.table_off if t16 is not None else 0,
-             t16.numel() if t16 is not None else 0, ptr(self.ns), s)
+             t16.numel() if t16 is not None else 0, ptr(self.run), s)
 
     # ------------------------------------------------------------------- driver
     def step(self, lr, step_size, use_graph=True, inputs=None, next_step_size=None):
@@ -531,7 +612,7 @@ class SyncFreeStep:
                 self._part_march("keep")
             self.graph_given = g
         self.step_size.fill_(float(step_size))
-        self._load(*inputs, stratified=False)
+        self._load_given(inputs)
         self.graph_given.replay()
         self.graph[1].replay()
         e = self._hash_events() if self.time_hash_bwd else None
@@ -550,12 +631,35 @@ class SyncFreeStep:
         if self.side is None:
             self.side = torch.cuda.Stream(device=self.dev)
         self.side.wait_event(done)
+        # the sampler's generator as of this sampling, for a checkpoint taken
+        # while the march is pending (it resumes by sampling these patches)
+        self._rng_pending = self._rng_state()
         with torch.cuda.stream(self.side):
             self.step_size.fill_(step_size)
             self.graph[0].replay()
             ev = torch.cuda.Event()
             ev.record(self.side)
         self._pending = (ev, step_size)
+
+    def _rng_state(self):
+        return self.gen.get_state() if self.gen is not None else \
+            torch.cuda.get_rng_state(self.dev)
+
+    def sampler_rng_state(self):
+        """State of the patch sampler's generator (its own, or torch's default
+        one) as of the next iteration's sampling, also when that iteration's
+        march already ran ahead (then the state before it sampled)."""
+        if self._pending is not None:
+            return self._rng_pending
+        return self._rng_state()
+
+    def set_sampler_rng_state(self, state):
+        if self._pending is not None:
+            raise RuntimeError("a march is pending; drain() it first")
+        if self.gen is not None:
+            self.gen.set_state(state)
+        else:
+            torch.cuda.set_rng_state(state, self.dev)
 
     def sync_step_count(self):
         """Sets the host's optimiser step count (flat.step_count) to the device's,
@@ -569,6 +673,12 @@ class SyncFreeStep:
     def skipped_iterations(self):
         """Iterations skipped so far for want of visible samples (reads the device)."""
         return int(self.n_skipped)
+
+    @property
+    def nonfinite_iterations(self):
+        """Iterations so far skipped for a non-finite loss or gradient (reads
+        the device; also counted in skipped_iterations)."""
+        return int(self.n_nonfinite)
 
     @property
     def capped_rays(self):

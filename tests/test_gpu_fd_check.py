@@ -1,0 +1,137 @@
+"""The hand-written backward and double-backward kernels against FP64 finite
+differences of the reference's loss, on a seeded subset of 64 patch rays'
+patches (64 patches of 3x3 rays) at the full-size workload's field (hash grid
+16 levels x 2 features, T = 2^19, all levels active; MLP 64 x 2 hidden):
+
+  device gradient: one iteration of the fp32 kernels on given inputs --
+    'dfd': the device iteration (SyncFreeStep, HIP graphs);
+    'ad':  the host-driven fused iteration (FusedStep), whose backward runs
+           the double backward through the analytic x-gradient
+           (sn_hashgrid_bwd_bwd + sn_mlp_bwd_ad);
+  reference: central differences, step 1e-6, of the CPU oracle's loss in
+    float64 (oracle/train_oracle.py: exp_runner.py:169-203 over
+    renderer.py:142-276 and fields.py:76-119), with the device's march and
+    visibility decision, for the 6 entries of largest gradient of each
+    parameter tensor (hash table, each layer's weight_v / weight_g / bias,
+    the variance).
+
+Tolerance: rtol 1e-3 (plus atol 1e-5 of the tensor's largest checked
+entry).  The oracle's own fp64 autograd gradient must agree with the finite
+differences to rtol 1e-6 (which pins the oracle's backward as well).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+N_PATCHES = 64
+PER_TENSOR = 6
+
+
+def _runner(dev, mode, method):
+    from supernormal_amd.runner import Runner, synthetic_conf
+    conf = synthetic_conf("tbs612", gradient_method=method, mlp_precision="fp32",
+                          step_mode=mode, batch_size=N_PATCHES)
+    r = Runner(conf, fused=True, device=dev, seed=0, rng_seed=0)
+    r.update_learning_rate()
+    for i in range(300):
+        r.train_iteration(i, ahead=False)
+    if r.sf_step is not None:
+        r.sf_step.drain()
+    while r.sdf_network.active_levels() < r.sdf_network.encoding.grid["n_levels"]:
+        r.sdf_network.increase_bandwidth()
+    torch.cuda.synchronize()
+    return r
+
+
+def _inputs(r, dev):
+    g = torch.Generator(device=dev).manual_seed(77)
+    o, d, pn, vi, nt, m = r.dataset.gen_random_patches(N_PATCHES, generator=g)
+    near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+    step = float(r.renderer.sampling_step_size)
+    u = torch.rand(near.shape, generator=g, device=dev, dtype=torch.float32)
+    return (o, d, pn, vi, nt, m, near + u * step, far), step
+
+
+def _device_grads(r):
+    net, var = r.sdf_network, r.deviation_network
+    out = {"params": net.encoding.params.grad}
+    for l in range(net.num_layers - 1):
+        lin = getattr(net, f"lin{l}")
+        out[f"lin{l}.weight_v"] = lin.weight_v.grad
+        out[f"lin{l}.weight_g"] = lin.weight_g.grad
+        out[f"lin{l}.bias"] = lin.bias.grad
+    out["variance"] = var.variance.grad
+    return {k: v.detach().double().cpu().reshape(-1).clone() for k, v in out.items()}
+
+
+def _oracle_tensors(tr):
+    t = {"params": tr.field.params}
+    for l in range(len(tr.field.v)):
+        t[f"lin{l}.weight_v"] = tr.field.v[l]
+        t[f"lin{l}.weight_g"] = tr.field.g[l]
+        t[f"lin{l}.bias"] = tr.field.b[l]
+    t["variance"] = tr.variance
+    return t
+
+
+@pytest.mark.parametrize("method", ["dfd", "ad"])
+def test_kernel_gradients_vs_fp64_finite_differences(dev, method):
+    from oracle.train_oracle import OracleTrainer
+    mode = "graph" if method == "dfd" else "fused"
+    r = _runner(dev, mode, method)
+    inputs, step = _inputs(r, dev)
+    tr = OracleTrainer(r.sdf_network, float(r.deviation_network.variance.detach()),
+                       r.occ_resolution, {"lr": r.lr, "loss_type": r.loss_type},
+                       dtype=torch.float64)
+    tr.binary = r.renderer.occupancy_grid.binary.cpu().numpy().astype(np.uint8)
+    cpu = [t.cpu() for t in inputs]
+    if mode == "graph":
+        sf = r.sf_step
+        sf.step(r.lr, step, use_graph=True, inputs=inputs)
+        torch.cuda.synchronize()
+        keep = sf.keep[:int(sf.n0)].cpu().numpy().astype(bool)
+    else:
+        fs = r.fused_step
+        r.optimizer.zero_grad()
+        o, d, pn, vi, nt, m, t_min, far = inputs
+        res_d = fs.forward_backward(o, d, pn, vi, nt, m, t_min, far, step, stratified=False)
+        assert res_d.loss is not None
+        torch.cuda.synchronize()
+        keep = fs.last_keep.cpu().numpy().astype(bool)
+    gd = _device_grads(r)
+    kw = dict(cap=r.max_samples_per_ray, method=method, keep=keep)
+    res = tr.forward_backward(*cpu, step, **kw)
+    assert res is not None
+    print(f"\n[{method}] visible samples {res['ri'].size}, loss {res['loss']:.6e}", flush=True)
+    go = {k: v.grad.detach().double().reshape(-1).clone() for k, v in _oracle_tensors(tr).items()}
+    tensors = _oracle_tensors(tr)
+
+    def loss():
+        with torch.enable_grad():
+            return tr.forward_backward(*cpu, step, **kw)["loss"]
+
+    h = 1e-6
+    for name, t in tensors.items():
+        g_o = go[name]
+        idx = torch.argsort(g_o.abs(), descending=True)[:PER_TENSOR]
+        fd = []
+        for i in idx.tolist():
+            flat = t.data.view(-1)
+            x0 = float(flat[i])
+            flat[i] = x0 + h
+            lp = loss()
+            flat[i] = x0 - h
+            lm = loss()
+            flat[i] = x0
+            fd.append((lp - lm) / (2 * h))
+        fd = np.array(fd)
+        dev_g = gd[name][idx].numpy()
+        orc_g = g_o[idx].numpy()
+        scale = np.abs(fd).max()
+        rel = np.abs(dev_g - fd) / np.maximum(np.abs(fd), 1e-300)
+        print(f"[{method}] {name:16s} max|fd| {scale:.3e}  device rel err max {rel.max():.2e}  "
+              f"oracle rel err max {(np.abs(orc_g - fd) / np.abs(fd)).max():.2e}", flush=True)
+        np.testing.assert_allclose(orc_g, fd, rtol=1e-6, atol=1e-9 * scale)
+        np.testing.assert_allclose(dev_g, fd, rtol=1e-3, atol=1e-5 * scale)

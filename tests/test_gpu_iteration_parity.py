@@ -14,7 +14,9 @@ Bit-exact (integer / index work and the fp32 march recurrence):
 The keep decision of the visibility filter (T >= 1e-8, renderer.py:133) is
 compared with the oracle's own; it may differ only for samples whose
 transmittance lies within a stated band around 1e-8 (the SDF in fp16 / fp32
-against fp64), and the oracle then continues from the device's decision.
+against fp64: |ln T - ln 1e-8| < 0.05 for fp32, < 4 for the fp16 MLP, whose
+SDF errors of ~2e-4 are multiplied by inv_s in every alpha of the ray and add
+up along it), and the oracle then continues from the device's decision.
 Within stated tolerances (fp32 / fp16 MLP with fp16 table copy):
   the normal, mask and eikonal losses (relative);
   the gradient of every parameter tensor: relative 2-norm of the difference.
@@ -30,7 +32,7 @@ pytestmark = pytest.mark.gpu
 # (keep band in |ln T - ln 1e-8|, loss rtol, gradient relative-norm tolerance
 #  for the hash table and for the MLP / variance tensors)
 TOL = {"fp32": dict(band=0.05, loss=2e-3, table=2e-2, mlp=2e-2),
-       "fp16": dict(band=0.5, loss=3e-2, table=1e-1, mlp=1e-1)}
+       "fp16": dict(band=4.0, loss=3e-2, table=1e-1, mlp=1e-1)}
 
 
 def _trained_runner(dev, precision):
@@ -96,8 +98,10 @@ def test_device_iteration_matches_oracle(dev, precision):
     diff = keep_dev != res["keep_oracle"]
     lnT = np.log(np.maximum(res["T0"], 1e-300))
     near_thr = np.abs(lnT - math.log(1e-8)) < tol["band"]
+    dist = np.abs(lnT - math.log(1e-8))
     print(f"[{precision}] keep decisions differing: {int(diff.sum())} of {n0} "
-          f"({int(near_thr.sum())} samples in the band)", flush=True)
+          f"({int(near_thr.sum())} samples in the band); largest |ln T - ln 1e-8| of a "
+          f"differing decision {float(dist[diff].max()) if diff.any() else 0.0:.3f}", flush=True)
     assert not (diff & ~near_thr).any()
     # the visible samples: bit-exact
     assert ns == res["ri"].size

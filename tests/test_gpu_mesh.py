@@ -79,9 +79,120 @@ def test_random_field_all_cases(dev):
     _same_triangles(got, ref)
 
 
-def test_extract_geometry_of_sdf(dev):
+@pytest.mark.parametrize("mesher", ["mc", "mt"])
+def test_extract_geometry_of_sdf(dev, mesher):
+    """extract_geometry (renderer.py:27-34: the mesh of -sdf = 0): marching cubes
+    gives an indexed mesh with one vertex per crossing grid edge (a closed
+    sphere: V - E + F = 2), marching tetrahedra a triangle soup."""
     from supernormal_amd.mesh import extract_geometry
     sdf = lambda p: p.norm(dim=-1) - 0.5
-    v, f = extract_geometry([-1, -1, -1], [1, 1, 1], 64, 0.0, lambda p: -sdf(p), device=dev)
-    assert f.shape[0] > 1000 and v.shape[0] == 3 * f.shape[0]
+    v, f = extract_geometry([-1, -1, -1], [1, 1, 1], 64, 0.0, lambda p: -sdf(p), device=dev,
+                            mesher=mesher)
+    assert f.shape[0] > 1000
     assert np.abs(np.linalg.norm(v, axis=1) - 0.5).max() < 2.0 / 63
+    if mesher == "mt":
+        assert v.shape[0] == 3 * f.shape[0]
+    else:
+        edges = {tuple(sorted((t[i], t[(i + 1) % 3]))) for t in f for i in range(3)}
+        assert len(v) - len(edges) + len(f) == 2
+        # outward (towards -sdf < 0): positive signed volume
+        w = v[f]
+        vol = np.einsum("ij,ij->i", w[:, 0], np.cross(w[:, 1], w[:, 2])).sum() / 6.0
+        assert abs(vol - 4.0 / 3.0 * np.pi * 0.125) < 0.01
+
+
+# ---------------------------------------------------------------------------
+# marching cubes (sn_mc_count / sn_mc_emit) and the mesh ray-cast
+# (sn_mesh_first_hit) against oracle/mesh_oracle.py
+# ---------------------------------------------------------------------------
+def _mc_gpu(dev, u, iso, bmin, bmax):
+    from supernormal_amd.mesh import marching_cubes_device
+    return marching_cubes_device(torch.as_tensor(u, device=dev), iso, bmin, bmax)
+
+
+@pytest.mark.parametrize("shape,iso", [((9, 11, 7), 0.1), ((16, 16, 16), -0.3)])
+def test_marching_cubes_matches_oracle(dev, shape, iso):
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(shape).astype(np.float32)
+    bmin, bmax = [-1.0, -2.0, 0.0], [1.0, 1.0, 3.0]
+    m = _mc_gpu(dev, u, iso, bmin, bmax)
+    ref_t, ref_ids = mesh_oracle.marching_cubes(u, iso)
+    R = np.array(shape, dtype=np.float64)
+    ref_w = ref_t / (R - 1.0) * (np.array(bmax) - np.array(bmin)) + np.array(bmin)
+    got = m.tris.cpu().numpy()
+    assert got.shape == ref_w.shape and got.shape[0] > 100
+    # same triangles in the same order (cube x-major, table order), same vertex ids
+    np.testing.assert_array_equal(m.vertex_ids.cpu().numpy(), ref_ids)
+    np.testing.assert_allclose(got, ref_w, atol=2e-6, rtol=0)
+    # vertices shared between cubes are bit-identical: one position per id
+    v, f = m.indexed()
+    again = v[f]
+    np.testing.assert_array_equal(again.astype(np.float32), got)
+
+
+def test_marching_cubes_sphere_closed(dev):
+    r, rad = 65, 0.6
+    ax = np.linspace(-1, 1, r)
+    x, y, z = np.meshgrid(ax, ax, ax, indexing="ij")
+    u = (np.sqrt(x * x + y * y + z * z) - rad).astype(np.float32)
+    m = _mc_gpu(dev, u, 0.0, [-1, -1, -1], [1, 1, 1])
+    v, f = m.indexed()
+    edges = {}
+    for t in f:
+        for i in range(3):
+            e = (t[i], t[(i + 1) % 3])
+            edges[e] = edges.get(e, 0) + 1
+    assert all(c == 1 and edges.get((b, a), 0) == 1 for (a, b), c in edges.items())
+    w = v[f]
+    vol = np.einsum("ij,ij->i", w[:, 0], np.cross(w[:, 1], w[:, 2])).sum() / 6.0
+    assert abs(vol - 4.0 / 3.0 * np.pi * rad ** 3) < 0.01
+    # Euler characteristic of a sphere
+    assert len(v) - len(edges) // 2 + len(f) == 2
+
+
+def test_mesh_first_hit_matches_bruteforce(dev):
+    from supernormal_amd.mesh import mesh_first_hits
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((12, 10, 14)).astype(np.float32)
+    bmin, bmax = [-1.0, -1.0, -1.0], [1.0, 0.8, 1.2]
+    m = _mc_gpu(dev, u, 0.0, bmin, bmax)
+    tris = m.tris.cpu().numpy().astype(np.float64)
+    n = 3000
+    o = rng.normal(size=(n, 3))
+    o = o / np.linalg.norm(o, axis=1, keepdims=True) * 3.0
+    d = rng.uniform(-0.9, 0.9, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    near = np.full(n, 0.5)
+    far = np.full(n, 5.0)
+    t_ref, tri_ref = mesh_oracle.ray_mesh_first_hit(o, d, tris, near, far)
+    t, tri = mesh_first_hits(m, torch.as_tensor(o, dtype=torch.float32, device=dev),
+                             torch.as_tensor(d, dtype=torch.float32, device=dev),
+                             torch.as_tensor(near, dtype=torch.float32, device=dev),
+                             torch.as_tensor(far, dtype=torch.float32, device=dev))
+    t, tri = t.cpu().numpy().astype(np.float64), tri.cpu().numpy()
+    hit_ref = tri_ref >= 0
+    print(f"rays {n}: oracle hits {hit_ref.sum()}, device hits {(tri >= 0).sum()}", flush=True)
+    # fp32 against fp64: decisions may differ only for rays grazing an edge;
+    # allow a handful, and equal hit points elsewhere
+    assert ((tri >= 0) != hit_ref).sum() <= 3
+    both = hit_ref & (tri >= 0)
+    assert np.abs(t[both] - t_ref[both]).max() < 1e-4
+    assert (tri[both] == tri_ref[both]).mean() > 0.99
+
+
+def test_visible_mesh_points_of_sphere(dev):
+    """The reference's evaluation path on an analytic sphere: marching cubes of
+    -sdf at 0 (renderer.py:27-34), first hits of the training rays
+    (exp_runner.py:618-630): the hits lie on the sphere."""
+    from supernormal_amd.mesh import extract_fields_device, marching_cubes_device, \
+        visible_mesh_points
+    from supernormal_amd.scene import config_dataset
+    ds = config_dataset("sphere128", dev, 0)
+    rad = 0.5
+    u = extract_fields_device([-1, -1, -1], [1, 1, 1], 129,
+                              lambda p: -(p.norm(dim=-1) - rad), dev)
+    m = marching_cubes_device(-u, 0.0, [-1, -1, -1], [1, 1, 1])
+    pts = visible_mesh_points(m, ds, rays_per_view=2000)
+    assert pts.shape[0] > 1000
+    rn = np.linalg.norm(pts, axis=1)
+    assert np.abs(rn - rad).max() < 2.0 / 128

@@ -176,3 +176,72 @@ def test_march_ahead_step_size_must_match(dev):
     sf.step(r.lr, 4e-3, use_graph=True)
     torch.cuda.synchronize()
     assert not sf.march_pending
+
+
+def _runner_ad(dev, step_mode):
+    from supernormal_amd.runner import Runner, synthetic_conf
+    conf = synthetic_conf("sphere128", gradient_method="ad", max_samples_per_ray=32,
+                          mlp_precision="fp32", step_mode=step_mode)
+    r = Runner(conf, fused=True, device=dev, seed=3)
+    r.update_learning_rate()
+    for i in range(12):
+        r.train_iteration(i, ahead=i < 11)
+    torch.cuda.synchronize()
+    return r
+
+
+def test_sync_free_ad_matches_fused_step(dev, monkeypatch):
+    """'ad' (analytic normals, double backward through the x-gradient) on the
+    device iteration against FusedStep's autograd over the same kernels."""
+    r = _runner_ad(dev, "sync_free")
+    fs, flat = r.fused_step, r.optimizer
+    from supernormal_amd.step import SyncFreeStep
+    sf = SyncFreeStep(r.sdf_network, r.deviation_network, r.renderer.occupancy_grid, flat,
+                      r.dataset, r.batch_size, r.patch_size, r.max_samples_per_ray,
+                      r.normal_weight, r.eikonal_weight, r.mask_weight, r.loss_type,
+                      precision="fp32", device=dev, half_table=False, method="ad")
+    _device_packing(r, sf, monkeypatch)
+    g = torch.Generator(device=dev).manual_seed(8)
+    o, d, pn, vi, nt, m = r.dataset.gen_random_patches(r.batch_size, generator=g)
+    near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+    step = 5e-3
+    flat.zero_grad()
+    res = fs.forward_backward(o, d, pn, vi, nt, m, near, far, step, stratified=False)
+    flat.collect()
+    g_ref = flat.grad.clone()
+    loss_ref = float(res.loss)
+    losses = sf.step(r.lr, step, use_graph=False, inputs=(o, d, pn, vi, nt, m))
+    torch.cuda.synchronize()
+    assert int(sf.ns) == res.n_samples and int(sf.n0) == res.n_marched
+    loss = float(losses[0] + losses[1] + losses[2])
+    print(f"\nad: loss {loss:.8e} fused {loss_ref:.8e}", flush=True)
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref) + 1e-7
+    scale = float(g_ref.abs().max())
+    torch.testing.assert_close(flat.grad, g_ref, rtol=1e-4, atol=1e-5 * scale)
+
+
+def test_graph_ad_replays_match_eager(dev):
+    r = _runner_ad(dev, "graph")
+    sf, flat = r.sf_step, r.optimizer
+    lr, step = r.lr, 5e-3
+    g = torch.Generator(device=dev).manual_seed(9)
+    for k in range(3):
+        inputs = r.dataset.gen_random_patches(r.batch_size, generator=g)
+        before = _state(flat)
+        t0 = int(flat.step_count)
+        sf.step(lr, step, use_graph=True, inputs=inputs)
+        torch.cuda.synchronize()
+        after, g_graph = _state(flat), flat.grad.clone()
+        lg = sf.losses.clone()
+        for t, b in zip((flat.data, flat.exp_avg, flat.exp_avg_sq), before):
+            t.copy_(b)
+        flat.step_count = t0
+        sf.adam_t.fill_(t0)
+        sf._adam_t_host = t0
+        sf.step(lr, step, use_graph=False, inputs=inputs)
+        torch.cuda.synchronize()
+        torch.testing.assert_close(sf.losses, lg, rtol=1e-5, atol=1e-7)
+        scale = float(g_graph.abs().max())
+        torch.testing.assert_close(flat.grad, g_graph, rtol=1e-4, atol=1e-5 * scale)
+        for t, a in zip((flat.data, flat.exp_avg, flat.exp_avg_sq), after):
+            torch.testing.assert_close(t, a, rtol=1e-4, atol=1e-6)
