@@ -50,6 +50,17 @@ int sn_pack_info(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
                  int32_t* packed_info, int32_t* n_bad, void* workspace,
                  size_t workspace_bytes, void* stream);
 
+/* pack_info that allocates nothing and takes no workspace: no counterpart   */
+/* in the reference; for callers that cannot call hipMalloc (code under HIP  */
+/* graph capture, tight serving loops, harnesses that forbid allocation).    */
+/* Same result as sn_pack_info.  Its scratch is a fixed device array, one    */
+/* per GPU for the life of the process, of sn_pack_info_workspace_bytes(     */
+/* sn_pack_info_noalloc_max_rays()) bytes; n_rays above that limit (2^22) is */
+/* SN_EINVAL.  Calls on one GPU share the array: overlapping calls race.     */
+int64_t sn_pack_info_noalloc_max_rays(void);
+int sn_pack_info_noalloc(const int64_t* ray_indices, int64_t n_samples, int32_t n_rays,
+                         int32_t* packed_info, int32_t* n_bad, void* stream);
+
 /* Exclusive scan of int32 counts into packed_info rows (base, count); also  */
 /* writes the grand total to *total (device int32, nullable).  This is the   */
 /* cumsum step of csrc/ray_marching.cu:257-258.                             */

@@ -24,6 +24,8 @@ SIGNATURES = {
     "sn_abi_version": (C.c_int, []),
     "sn_pack_info_workspace_bytes": (_sz, [_i32]),
     "sn_pack_info": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _sz, _p]),
+    "sn_pack_info_noalloc_max_rays": (_i64, []),
+    "sn_pack_info_noalloc": (C.c_int, [_p, _i64, _i32, _p, _p, _p]),
     "sn_scan_workspace_bytes": (_sz, [_i64]),
     "sn_fill_span_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _i32, _f32, _p]),
     "sn_counts_to_packed_info": (C.c_int, [_p, _i32, _p, _p, _p, _sz, _p]),
