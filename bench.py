@@ -65,6 +65,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "tbs612": "torus+box+3 spheres, 20 views 612x512, hash 16x2 T=2^19 base 16 x1.38, MLP 64x2, "
+              "2048 3x3 patches (18432 rays), occupancy 128^3",
+    "prims1024": "8 random primitives with 0.005 sinusoidal ridges, 50 views 1024x1024, hash "
+                 "16x2 T=2^21 base 16 x1.38, MLP 64x2, 8192 3x3 patches (73728 rays), "
+                 "occupancy 256^3",
+    "sphere128": "sphere r 0.5, 8 views 128x128, hash 8x2 T=2^15 base 16 x1.5, MLP 32x1, 256 3x3 "
+                 "patches, occupancy 64^3",
+}
 MFMA_PEAK = {"fp16": 2.5e15, "fp32": 157.3e12}   # dense fp16/bf16, fp32 MFMA (FLOP/s)
 
 
@@ -475,7 +484,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed iterations (end of schedule)")
     ap.add_argument("--warmup", type=int, default=20, help="untimed iterations before them")
-    ap.add_argument("--workload", default="tbs612")
+    ap.add_argument("--workload", default="tbs612", choices=list(WORKLOADS))
     ap.add_argument("--method", default="dfd", choices=["dfd", "ad"])
     ap.add_argument("--mode", default="graph", choices=["fused", "sync_free", "graph"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"])
@@ -520,9 +529,8 @@ def main(argv=None):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if a.precision == "fp32" else "fp16/bf16 MFMA + f32",
         "data": "synthetic",
-        "config": {"workload": f"{a.workload}: torus+box+3 spheres, 20 views 612x512, hash 16x2 "
-                               f"T=2^19 base 16 x1.38, MLP 64x2, 2048 3x3 patches (18432 rays), "
-                               f"occupancy 128^3, {runner.max_samples_per_ray} samples/ray "
+        "config": {"workload": f"{a.workload}: {WORKLOADS[a.workload]}, "
+                               f"{runner.max_samples_per_ray} samples/ray "
                                f"capacity, gradient {a.method}, mode {mode}, mlp {a.precision}; "
                                f"timed: the last {a.steps} of {res['iterations']} iterations "
                                f"after {a.warmup} untimed",

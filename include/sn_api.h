@@ -459,6 +459,36 @@ int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float
 int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_dev,
                     void* stream);
 
+/* Adam steps of several flat parameter buffers in one launch (the serving   */
+/* loop's resident jobs; no counterpart in the reference, whose one          */
+/* optimiser is torch.optim.Adam, exp_runner.py:97, 207).  segments: a DEVICE */
+/* array of n_segments SnAdamSegment in block order: segment s owns blocks   */
+/* [block0, block0 + n_blocks) of a launch of total_blocks blocks; n_blocks  */
+/* = sn_adam_multi_blocks(n).  Each segment reads its learning rate and step */
+/* count (both required: a segment without them is left alone) and run flag */
+/* (nullable: always run) from device memory, exactly as                    */
+/* sn_adam_step_copy16 with lr_dev / step_dev / run_dev; copy16 (nullable)   */
+/* is its fp16 copy of [copy_off, copy_off + copy_n).  Buffers 16-byte        */
+/* aligned.                                                                   */
+typedef struct SnAdamSegment {
+  float* params;
+  const float* grads;
+  float* exp_avg;
+  float* exp_avg_sq;
+  int64_t n;
+  const float* lr_dev;
+  const int32_t* step_dev;
+  const int32_t* run_dev;
+  void* copy16;
+  int64_t copy_off;
+  int64_t copy_n;
+  int32_t block0;
+  int32_t n_blocks;
+} SnAdamSegment;
+int32_t sn_adam_multi_blocks(int64_t n);
+int sn_adam_multi(const SnAdamSegment* segments, int32_t n_segments, int32_t total_blocks,
+                  double beta1, double beta2, double eps, void* stream);
+
 /* Step guard (no counterpart in the reference, which has no NaN check):     */
 /* run = 1 if *n_samples > 0, the n_losses losses are finite and every one    */
 /* of the n gradients is finite, else 0; pass run to sn_step_advance and     */
@@ -583,6 +613,7 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                         const int32_t* n_active_dev, const float* grad_enc,
                            float* grad_params, float* grad_x, void* workspace,
                            size_t workspace_bytes, void* stream);
+
 int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                             const float* params, int32_t n_levels, int32_t n_features,
                             int32_t log2_hashmap_size, int32_t base_resolution,

@@ -270,17 +270,26 @@ void or_occ_ema_max(float* occs, int32_t n_cells, const int64_t* idx, const floa
   free(mx);
 }
 
-/* binary = occs > min(mean(occs), thre) (grid.py:237-239); mean = fp64 sum over
- * 1024-cell chunks, each sequential, chunks summed sequentially. */
+/* compensated (Kahan) fp32 accumulation: s + x with the running error c */
+static void kahan_add(float* s, float* c, float x) {
+  const float y = x - *c;
+  const float t = *s + y;
+  *c = (t - *s) - y;
+  *s = t;
+}
+
+/* binary = occs > min(mean(occs), thre) (grid.py:237-239); mean = compensated
+ * fp32 sum over 1024-cell chunks, each sequential, chunk sums summed
+ * sequentially (compensated too), divided by n_cells in fp32. */
 float or_occ_binarize(const float* occs, int32_t n_cells, float thre, uint8_t* binary) {
-  double s = 0.0;
+  float s = 0.f, sc = 0.f;
   for (int64_t b = 0; b < n_cells; b += 1024) {
-    double p = 0.0;
+    float p = 0.f, pc = 0.f;
     int64_t e = b + 1024 < n_cells ? b + 1024 : n_cells;
-    for (int64_t c = b; c < e; ++c) p += (double)occs[c];
-    s += p;
+    for (int64_t c = b; c < e; ++c) kahan_add(&p, &pc, occs[c]);
+    kahan_add(&s, &sc, p);
   }
-  const float mean = (float)(s / (double)n_cells);
+  const float mean = s / (float)n_cells;
   const float t = fminf(mean, thre);
   for (int32_t c = 0; c < n_cells; ++c) binary[c] = occs[c] > t;
   return t;

@@ -93,6 +93,8 @@ SIGNATURES = {
     "sn_step_advance": (C.c_int, [_p, _p, _p, _p]),
     "sn_to_half": (C.c_int, [_p, _p, _i64, _p]),
     "sn_step_guard": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p]),
+    "sn_adam_multi_blocks": (_i32, [_i64]),
+    "sn_adam_multi": (C.c_int, [_p, _i32, _i32, C.c_double, C.c_double, C.c_double, _p]),
     "sn_hashgrid_fwd16_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p]),
     "sn_pack_info_dev": (C.c_int, [_p, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sn_compact_samples_dev": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32,
@@ -113,6 +115,14 @@ SIGNATURES = {
     "sn_occ_sample_workspace_bytes": (_sz, [_i32]),
     "sn_occ_sample_cells": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _sz, _p]),
 }
+
+class SnAdamSegment(C.Structure):
+    """One row of sn_adam_multi's segment table (include/sn_api.h)."""
+    _fields_ = [("params", _p), ("grads", _p), ("exp_avg", _p), ("exp_avg_sq", _p),
+                ("n", _i64), ("lr_dev", _p), ("step_dev", _p), ("run_dev", _p),
+                ("copy16", _p), ("copy_off", _i64), ("copy_n", _i64), ("block0", _i32),
+                ("n_blocks", _i32)]
+
 
 _lib = None
 

@@ -10,10 +10,13 @@
 namespace sn {
 
 // 1 - beta is formed in double on the host: in fp32, 1 - 0.999f is off by
-// 1.3e-5 relative, which torch (Python floats) does not do.
+// 1.3e-5 relative, which torch (Python floats) does not do.  The bias
+// corrections of a step count held on the device are 1 - beta^t =
+// -expm1(t ln beta) in fp32 (ln beta from the host's double): accurate for
+// small t, where 1 - exp would cancel.
 struct AdamK {
   float b1, b2, omb1, omb2, eps, step_size, inv_sqrt_bc2;
-  double b1d, b2d;
+  float ln_b1, ln_b2;
 };
 
 __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, const AdamK& k) {
@@ -35,9 +38,9 @@ adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restric
   // (the reference's iteration never reaches its optimiser step then)
   if (run_dev && *run_dev <= 0) return;  // uniform
   if (step_dev) {  // bias corrections of the step held on the device
-    const double st = (double)*step_dev;
-    bc1 = (float)(1.0 - pow(k.b1d, st));
-    k.inv_sqrt_bc2 = (float)(1.0 / sqrt(1.0 - pow(k.b2d, st)));
+    const float st = (float)*step_dev;
+    bc1 = -expm1f(st * k.ln_b1);
+    k.inv_sqrt_bc2 = 1.f / sqrtf(-expm1f(st * k.ln_b2));
   }
   if (lr_dev) k.step_size = lr_dev[0] / bc1;
   else k.step_size = lr_scalar / bc1;
@@ -68,6 +71,63 @@ adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restric
   for (int64_t i = n4 * 4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
     adam1(p[i], g[i], m[i], v[i], k);
     if (copy16 && i - copy_off >= 0 && i - copy_off < copy_n) copy16[i - copy_off] = (_Float16)p[i];
+  }
+}
+
+// Several optimisers' Adam steps in one launch (the serving loop's resident
+// jobs): segment s owns blocks [block0, block0 + n_blocks) of the grid and
+// runs adam_kernel's body over its own flat buffers with its own learning
+// rate, step count, run flag and fp16 copy.
+__global__ void __launch_bounds__(256)
+adam_multi_kernel(const SnAdamSegment* __restrict__ segs, int32_t n_segs, AdamK k) {
+  const int b = blockIdx.x;
+  int si = 0;
+  while (si + 1 < n_segs && segs[si + 1].block0 <= b) ++si;  // uniform
+  const SnAdamSegment sg = segs[si];
+  if (b >= sg.block0 + sg.n_blocks || !sg.step_dev || !sg.lr_dev) return;  // uniform
+  if (sg.run_dev && *sg.run_dev <= 0) return;
+  float bc1 = 1.f;
+  {
+    const float st = (float)*sg.step_dev;
+    bc1 = -expm1f(st * k.ln_b1);
+    k.inv_sqrt_bc2 = 1.f / sqrtf(-expm1f(st * k.ln_b2));
+  }
+  k.step_size = sg.lr_dev[0] / bc1;
+  float* p = sg.params;
+  const float* g = sg.grads;
+  float* m = sg.exp_avg;
+  float* v = sg.exp_avg_sq;
+  const int64_t n = sg.n;
+  _Float16* copy16 = reinterpret_cast<_Float16*>(sg.copy16);
+  const int64_t n4 = n / 4;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  const float4* g4 = reinterpret_cast<const float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const int64_t stride = (int64_t)sg.n_blocks * blockDim.x;
+  const int64_t first = (int64_t)(b - sg.block0) * blockDim.x + threadIdx.x;
+  for (int64_t i = first; i < n4; i += stride) {
+    float4 pp = p4[i], gg = g4[i], mm = m4[i], vv = v4[i];
+    adam1(pp.x, gg.x, mm.x, vv.x, k);
+    adam1(pp.y, gg.y, mm.y, vv.y, k);
+    adam1(pp.z, gg.z, mm.z, vv.z, k);
+    adam1(pp.w, gg.w, mm.w, vv.w, k);
+    p4[i] = pp;
+    m4[i] = mm;
+    v4[i] = vv;
+    if (copy16) {
+      const float q[4] = {pp.x, pp.y, pp.z, pp.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t c = 4 * i + e - sg.copy_off;
+        if (c >= 0 && c < sg.copy_n) copy16[c] = (_Float16)q[e];
+      }
+    }
+  }
+  for (int64_t i = n4 * 4 + first; i < n; i += stride) {
+    adam1(p[i], g[i], m[i], v[i], k);
+    if (copy16 && i - sg.copy_off >= 0 && i - sg.copy_off < sg.copy_n)
+      copy16[i - sg.copy_off] = (_Float16)p[i];
   }
 }
 
@@ -153,6 +213,31 @@ int sn_step_advance(int32_t* step_dev, const int32_t* run_dev, int32_t* skipped_
   return check_launch("sn_step_advance");
 }
 
+int32_t sn_adam_multi_blocks(int64_t n) {
+  return n <= 0 ? 0 : (int32_t)grid_for(ceil_div(n, 4), 256, 256 * 8);
+}
+
+int sn_adam_multi(const SnAdamSegment* segments, int32_t n_segments, int32_t total_blocks,
+                  double beta1, double beta2, double eps, void* stream) {
+  clear_error();
+  SN_REQUIRE(n_segments >= 0 && total_blocks >= 0, "sn_adam_multi: negative size");
+  if (n_segments == 0 || total_blocks == 0) return SN_OK;
+  SN_REQUIRE(segments, "sn_adam_multi: null segment table");
+  AdamK k;
+  k.b1 = (float)beta1;
+  k.b2 = (float)beta2;
+  k.omb1 = (float)(1.0 - beta1);
+  k.omb2 = (float)(1.0 - beta2);
+  k.eps = (float)eps;
+  k.ln_b1 = (float)std::log(beta1);
+  k.ln_b2 = (float)std::log(beta2);
+  k.inv_sqrt_bc2 = 1.f;
+  k.step_size = 0.f;
+  adam_multi_kernel<<<(unsigned)total_blocks, 256, 0, as_stream(stream)>>>(segments, n_segments,
+                                                                          k);
+  return check_launch("sn_adam_multi");
+}
+
 int sn_step_guard(const int32_t* n_samples, const float* losses, int32_t n_losses,
                   const float* grads, int64_t n, int32_t* run, int32_t* n_nonfinite,
                   void* stream) {
@@ -201,8 +286,8 @@ int sn_adam_step_copy16(float* params, const float* grads, float* exp_avg, float
   k.omb1 = (float)(1.0 - beta1);
   k.omb2 = (float)(1.0 - beta2);
   k.eps = (float)eps;
-  k.b1d = beta1;
-  k.b2d = beta2;
+  k.ln_b1 = (float)std::log(beta1);
+  k.ln_b2 = (float)std::log(beta2);
   const double st = step >= 1 ? (double)step : 1.0;
   const double bc1 = 1.0 - std::pow(beta1, st);
   const double bc2 = 1.0 - std::pow(beta2, st);

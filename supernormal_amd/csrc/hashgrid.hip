@@ -784,6 +784,45 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
   }
 }
 
+// round(v * 2^K) as a 64-bit integer (round half to even, the result of
+// llrint on the exact product), from the bits of v: v = m * 2^(e - 150), so
+// v * 2^K = m * 2^(e - 150 + K), a shift of the 24-bit significand.  The
+// caller guarantees |v| * 2^K < 2^62 and finite v.  Integer ops only.
+__device__ __forceinline__ long long fixed_of(float v, int K) {
+  const uint32_t b = __float_as_uint(v);
+  const int e = (int)((b >> 23) & 0xffu);
+  uint32_t m = b & 0x7fffffu;
+  int sh;
+  if (e) {
+    m |= 0x800000u;
+    sh = e - 150 + K;
+  } else {
+    sh = -149 + K;  // subnormal: m * 2^-149
+  }
+  unsigned long long q;
+  if (sh >= 0) {
+    q = (unsigned long long)m << sh;
+  } else if (sh < -25) {
+    q = 0ull;  // below half of the last unit
+  } else {
+    const int r = -sh;
+    const uint32_t half = 1u << (r - 1);
+    const uint32_t rem = m & ((1u << r) - 1u);
+    uint32_t t = m >> r;
+    if (rem > half || (rem == half && (t & 1u))) ++t;
+    q = t;
+  }
+  return (b >> 31) ? -(long long)q : (long long)q;
+}
+
+// 2^e with count * vmax < 2^e (e from the fp32 product; rounding can only
+// raise it, which keeps the bound)
+__device__ __forceinline__ int bound_exp(float vmax, int32_t count) {
+  int e;
+  frexpf(vmax * (float)count, &e);
+  return e;
+}
+
 template <int F>
 __global__ void __launch_bounds__(1024)
 hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
@@ -813,10 +852,7 @@ hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
     for (int64_t j = tid; j < rows * F; j += 1024) tg[j] += accf[j];
     return;
   }
-  int e;
-  frexp((double)vmax * (double)nrec, &e);  // count * max|value| < 2^e
-  const int K = 61 - e;
-  const double scale = ldexp(1.0, K), inv = ldexp(1.0, -K);
+  const int K = 61 - bound_exp(vmax, nrec);  // count * max|value| < 2^(61 - K)
   for (int j = tid; j < kBucketEntries * F; j += 1024) acc[j] = 0ull;
   __syncthreads();
   // kU records per lane in flight: their loads are issued before the first
@@ -838,23 +874,20 @@ hash_bwd_bucket_sum_kernel(HashGridDesc d, BucketDesc bd, BucketPool pool,
 #pragma unroll
     for (int u = 0; u < kU; ++u)
 #pragma unroll
-      for (int f = 0; f < F; ++f) {
-        const long long q = llrint((double)v[u][f] * scale);
-        atomicAdd(&acc[row[u] * F + f], (unsigned long long)q);
-      }
+      for (int f = 0; f < F; ++f)
+        atomicAdd(&acc[row[u] * F + f], (unsigned long long)fixed_of(v[u][f], K));
   }
   for (; r < nrec; r += 1024) {
     const int row = pool.row[at0 + r];
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      const long long q = llrint((double)pool.val[(at0 + r) * F + f] * scale);
-      atomicAdd(&acc[row * F + f], (unsigned long long)q);
-    }
+    for (int f = 0; f < F; ++f)
+      atomicAdd(&acc[row * F + f], (unsigned long long)fixed_of(pool.val[(at0 + r) * F + f], K));
   }
   __syncthreads();
+  // sum * 2^-K: the int64 -> fp32 conversion rounds once, the scaling is exact
   for (int64_t j = tid; j < rows * F; j += 1024) {
     const long long s = (long long)acc[j];
-    if (s) tg[j] += (float)((double)s * inv);
+    if (s) tg[j] += ldexpf((float)s, -K);
   }
 }
 

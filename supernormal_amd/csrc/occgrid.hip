@@ -5,8 +5,9 @@
 //   * scatter-max of the new occupancies goes through an order-preserving
 //     float -> uint32 key and integer atomicMax, so duplicate cell indices give
 //     the same result in any order (the reference's index_put is racy);
-//   * the grid mean is an fp64 sum over fixed 1024-cell chunks, sequential
-//     inside a chunk and across chunks, identical to the C oracle.
+//   * the grid mean is a compensated (Kahan) fp32 sum over fixed 1024-cell
+//     chunks, sequential inside a chunk and across chunks, identical to the
+//     C oracle (no fp64 on the device).
 // Compiled with -ffp-contract=off.
 #include "sn_common.h"
 
@@ -69,27 +70,37 @@ ema_apply_kernel(float* __restrict__ occs, int32_t n_cells, const uint32_t* __re
   }
 }
 
+// Compensated (Kahan) fp32 summation in a fixed order: every 1024-cell chunk
+// sequentially, then the chunk sums sequentially; the order and the
+// operations (no contraction) are the oracle's, so the mean is bit-exact.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
 __global__ void __launch_bounds__(256)
-chunk_sum_kernel(const float* __restrict__ occs, int32_t n_cells, double* __restrict__ part) {
+chunk_sum_kernel(const float* __restrict__ occs, int32_t n_cells, float* __restrict__ part) {
   const int64_t ch = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t beg = ch * kMeanChunk;
   if (beg >= n_cells) return;
   int64_t end = beg + kMeanChunk;
   if (end > n_cells) end = n_cells;
-  double s = 0.0;
+  float s = 0.f, c = 0.f;
   if (end - beg == kMeanChunk && (reinterpret_cast<uintptr_t>(occs + beg) & 15) == 0) {
     // the same sequential order, 16 bytes per load
     const float4* v4 = reinterpret_cast<const float4*>(occs + beg);
 #pragma unroll 8
     for (int k = 0; k < kMeanChunk / 4; ++k) {
       const float4 v = v4[k];
-      s += (double)v.x;
-      s += (double)v.y;
-      s += (double)v.z;
-      s += (double)v.w;
+      kahan_add(s, c, v.x);
+      kahan_add(s, c, v.y);
+      kahan_add(s, c, v.z);
+      kahan_add(s, c, v.w);
     }
   } else {
-    for (int64_t c = beg; c < end; ++c) s += (double)occs[c];
+    for (int64_t k = beg; k < end; ++k) kahan_add(s, c, occs[k]);
   }
   part[ch] = s;
 }
@@ -98,20 +109,20 @@ chunk_sum_kernel(const float* __restrict__ occs, int32_t n_cells, double* __rest
 // then lane 0 adds them in order (the loads no longer wait one by one).
 constexpr int kThreThreads = 256, kThreTile = 4096;
 __global__ void __launch_bounds__(kThreThreads)
-thre_kernel(const double* __restrict__ part, int64_t n_chunks, int32_t n_cells, float thre,
+thre_kernel(const float* __restrict__ part, int64_t n_chunks, int32_t n_cells, float thre,
             float* __restrict__ thre_out) {
-  __shared__ double tile[kThreTile];
-  double s = 0.0;
+  __shared__ float tile[kThreTile];
+  float s = 0.f, c = 0.f;
   for (int64_t t0 = 0; t0 < n_chunks; t0 += kThreTile) {  // uniform
     const int64_t m = n_chunks - t0 < kThreTile ? n_chunks - t0 : kThreTile;
     for (int64_t i = threadIdx.x; i < m; i += kThreThreads) tile[i] = part[t0 + i];
     __syncthreads();
     if (threadIdx.x == 0)
-      for (int64_t i = 0; i < m; ++i) s += tile[i];
+      for (int64_t i = 0; i < m; ++i) kahan_add(s, c, tile[i]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const float mean = (float)(s / (double)n_cells);
+    const float mean = s / (float)n_cells;
     *thre_out = fminf(mean, thre);
   }
 }
@@ -152,7 +163,7 @@ int sn_occ_cell_points(const int64_t* cell_idx, int32_t n, const float* jitter, 
 size_t sn_occ_workspace_bytes(int32_t n_cells) {
   size_t a = align_up((size_t)(n_cells > 0 ? n_cells : 1) * sizeof(uint32_t), 256);
   size_t chunks = (size_t)ceil_div(n_cells > 0 ? n_cells : 1, kMeanChunk);
-  size_t b = align_up(chunks * sizeof(double), 256) + 256;
+  size_t b = align_up(chunks * sizeof(float), 256) + 256;
   return a > b ? a : b;
 }
 
@@ -187,9 +198,9 @@ int sn_occ_binarize(const float* occs, int32_t n_cells, float thre, uint8_t* bin
   }
   hipStream_t s = as_stream(stream);
   const int64_t n_chunks = ceil_div(n_cells, kMeanChunk);
-  double* part = reinterpret_cast<double*>(ws);
+  float* part = reinterpret_cast<float*>(ws);
   float* t_dev = thre_out;
-  if (!t_dev) t_dev = reinterpret_cast<float*>((char*)ws + align_up(n_chunks * 8, 256));
+  if (!t_dev) t_dev = reinterpret_cast<float*>((char*)ws + align_up(n_chunks * sizeof(float), 256));
   chunk_sum_kernel<<<(unsigned)ceil_div(n_chunks, 256), 256, 0, s>>>(occs, n_cells, part);
   thre_kernel<<<1, kThreThreads, 0, s>>>(part, n_chunks, n_cells, thre, t_dev);
   binarize_kernel<<<grid_for(n_cells, 256), 256, 0, s>>>(occs, n_cells, t_dev, binary);

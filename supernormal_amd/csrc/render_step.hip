@@ -314,14 +314,66 @@ render_step_kernel(RenderArgs a) {
 // are register arithmetic.  Along the samples, the transmittance is an
 // exclusive prefix product and the weight backward's recurrence
 //   Q_j = gw_{j+1} alpha_{j+1} + (1 - alpha_{j+1}) Q_{j+1}
-// a suffix scan of affine maps Q -> b + a Q, both as wavefront scans with
-// lane shuffles; chunks carry T forwards and Q backwards.  Same quantities
-// and outputs as render_step_kernel, summed in another order.
+// a suffix scan of affine maps Q -> b + a Q, both as wavefront scans; chunks
+// carry T forwards and Q backwards.  The scans and sums are DPP lane moves
+// (VALU, no LDS round trip), the backward walks a chunk's samples in reverse
+// lane order so that its suffix scan is a prefix scan.  Per-lane partial sums
+// are reduced once per patch.  Same quantities and outputs as
+// render_step_kernel, summed in another order.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+// DPP controls (gfx9): row_shr:n = 0x110 + n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143, wave_shr:1 = 0x138, wave_shl:1 = 0x130.  A lane whose
+// source lies outside the wavefront, or whose row is masked off, reads `old`.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dpp_or(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), kCtrl,
+                                                    kRowMask, 0xf, false));
+}
+
+// Inclusive scans over the 64 lanes: Kogge-Stone within rows of 16 lanes,
+// then the last lane of row 0 / rows 0-1 broadcast into the rows above.
+__device__ __forceinline__ float wave_incl_sum(float v) {
+  v += dpp_or<0x111>(0.f, v);
+  v += dpp_or<0x112>(0.f, v);
+  v += dpp_or<0x114>(0.f, v);
+  v += dpp_or<0x118>(0.f, v);
+  v += dpp_or<0x142, 0xa>(0.f, v);
+  v += dpp_or<0x143, 0xc>(0.f, v);
   return v;
+}
+
+__device__ __forceinline__ float wave_incl_prod(float v) {
+  v *= dpp_or<0x111>(1.f, v);
+  v *= dpp_or<0x112>(1.f, v);
+  v *= dpp_or<0x114>(1.f, v);
+  v *= dpp_or<0x118>(1.f, v);
+  v *= dpp_or<0x142, 0xa>(1.f, v);
+  v *= dpp_or<0x143, 0xc>(1.f, v);
+  return v;
+}
+
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Sum over the wavefront, the same value in every lane.
+__device__ __forceinline__ float wave_sum(float v) { return lane63(wave_incl_sum(v)); }
+
+// Affine maps x -> b + a x: lane l ends with M_l o M_(l-1) o ... o M_0.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ void affine_step(float& a, float& b) {
+  const float as = dpp_or<kCtrl, kRowMask>(1.f, a), bs = dpp_or<kCtrl, kRowMask>(0.f, b);
+  b = b + a * bs;
+  a = a * as;
+}
+
+__device__ __forceinline__ void wave_incl_affine(float& a, float& b) {
+  affine_step<0x111>(a, b);
+  affine_step<0x112>(a, b);
+  affine_step<0x114>(a, b);
+  affine_step<0x118>(a, b);
+  affine_step<0x142, 0xa>(a, b);
+  affine_step<0x143, 0xc>(a, b);
 }
 
 template <int Ph, int Pw, bool AD>
@@ -439,25 +491,27 @@ render_patch_kernel(RenderArgs a) {
     eval(j, act, q);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      // inclusive prefix product of (1 - alpha), then exclusive
-      float incl = 1.f - q.alpha[k];
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const float t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl *= t;
-      }
-      float excl = __shfl_up(incl, 1, 64);
-      if (lane == 0) excl = 1.f;
+      // inclusive prefix product of (1 - alpha), then exclusive (wave_shr:1)
+      const float incl = wave_incl_prod(1.f - q.alpha[k]);
+      const float excl = dpp_or<0x138>(1.f, incl);
       const float T = Tin[k] * excl;
       const float w = q.alpha[k] * T;
       if (act) a.g_sdf_s[((int64_t)base + j) * P + k] = T;  // parked for the backward
       const float nrm = sqrtf(q.g[k][0] * q.g[k][0] + q.g[k][1] * q.g[k][1] + q.g[k][2] * q.g[k][2]);
-      ws[k] += wave_sum(w);
+      // per-lane partial sums, reduced once after the last chunk
+      ws[k] += w;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) cn[k][c] += wave_sum(w * q.g[k][c]);
-      eik[k] += wave_sum(act ? (nrm - 1.f) * (nrm - 1.f) : 0.f);
-      Tin[k] *= __shfl(incl, 63, 64);
+      for (int c = 0; c < 3; ++c) cn[k][c] += w * q.g[k][c];
+      eik[k] += act ? (nrm - 1.f) * (nrm - 1.f) : 0.f;
+      Tin[k] *= lane63(incl);
     }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    ws[k] = wave_sum(ws[k]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cn[k][c] = wave_sum(cn[k][c]);
+    eik[k] = wave_sum(eik[k]);
   }
   // ---------------- losses of every ray and their gradients -----------------
   float g_cn[P][3], g_ws[P];
@@ -499,7 +553,8 @@ render_patch_kernel(RenderArgs a) {
   float g_inv = 0.f;
   const int n_chunks = (cnt + 63) / 64;
   for (int ch = n_chunks - 1; ch >= 0; --ch) {
-    const int j = ch * 64 + lane;
+    // reverse lane order: lane 0 holds the chunk's last sample
+    const int j = ch * 64 + 63 - lane;
     const bool act = j < cnt;
     const int64_t s = (int64_t)base + j;
     Q q;
@@ -518,21 +573,14 @@ render_patch_kernel(RenderArgs a) {
       for (int c = 0; c < 3; ++c) gg[c] = act ? (w * g_cn[k][c] + ek * q.g[k][c]) : 0.f;
       const float gw = act ? (g_ws[k] + g_cn[k][0] * q.g[k][0] + g_cn[k][1] * q.g[k][1] +
                               g_cn[k][2] * q.g[k][2]) : 0.f;
-      // suffix scan of the maps M_j(Q) = gw_j alpha_j + (1 - alpha_j) Q
+      // suffix scan of the maps M_j(Q) = gw_j alpha_j + (1 - alpha_j) Q: in
+      // reversed lane order a prefix scan, lane l = S_j (chunk end .. j)
       float ma = act ? 1.f - q.alpha[k] : 1.f, mb = act ? gw * q.alpha[k] : 0.f;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const float na = __shfl_down(ma, off, 64), nb = __shfl_down(mb, off, 64);
-        if (lane + off < 64) {
-          mb = mb + ma * nb;
-          ma = ma * na;
-        }
-      }
-      // Q_j = S_{j+1}(Qin) (lane 63: Qin)
-      float sa = __shfl_down(ma, 1, 64), sb = __shfl_down(mb, 1, 64);
-      if (lane == 63) { sa = 1.f; sb = 0.f; }
+      wave_incl_affine(ma, mb);
+      // Q_j = S_{j+1}(Qin): the previous lane's map (lane 0: the identity)
+      const float sa = dpp_or<0x138>(1.f, ma), sb = dpp_or<0x138>(0.f, mb);
       const float Qj = sb + sa * Qin[k];
-      const float Qnext = __shfl(mb + ma * Qin[k], 0, 64);  // S_{first}(Qin): carried back
+      const float Qnext = lane63(mb + ma * Qin[k]);  // S_{chunk start}(Qin): carried back
       const float g_alpha = act ? Tj * (gw - Qj) : 0.f;
       const float gr = (q.r[k] >= 0.f && q.r[k] <= 1.f) ? g_alpha : 0.f;
       const float den = q.c[k] + 1e-5f;
@@ -576,14 +624,15 @@ render_patch_kernel(RenderArgs a) {
         else { G[k + Pw] += ay; G[k - Pw] -= ay; }
       }
     }
-    // ends: own SDF row, or the next sample's start (added to that start)
+    // ends: own SDF row, or the next sample's start (added to that start);
+    // sample j - 1 sits in lane + 1 (wave_shl:1)
     const bool to_next = act && di < 0;
-    const int prev_to_next = __shfl_up(to_next ? 1 : 0, 1, 64);
+    const float prev_to_next = dpp_or<0x130>(0.f, to_next ? 1.f : 0.f);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      const float from_prev = __shfl_up(GE[k], 1, 64);
+      const float from_prev = dpp_or<0x130>(0.f, GE[k]);
       float gs = G[k];
-      if (lane > 0 && prev_to_next) gs += from_prev;
+      if (lane < 63 && prev_to_next != 0.f) gs += from_prev;
       if (act) {
         // the chunk's first sample may also receive from the previous chunk,
         // whose lane 63 adds its share later (read-modify-write below)
@@ -591,7 +640,7 @@ render_patch_kernel(RenderArgs a) {
         if (di >= 0) g_sdf_e[(int64_t)di * P + k] = GE[k];
       }
     }
-    if (lane == 63 && to_next && j + 1 < cnt) {
+    if (lane == 0 && to_next && j + 1 < cnt) {
       // sample j + 1 is the next chunk's first sample, already written
 #pragma unroll
       for (int k = 0; k < P; ++k) a.g_sdf_s[(s + 1) * P + k] += GE[k];

@@ -89,7 +89,7 @@ class _RenderStepFn(torch.autograd.Function):
         S, P, Nd = st["S"], st["P"], st["Nd"]
         g_ss = torch.empty(S * P, device=dev)
         g_se = torch.empty(max(Nd, 1) * P, device=dev)
-        g_gs = torch.empty(S * P * 3, device=dev) if st["ad"] else None
+        g_gs = torch.empty(S * P, 3, device=dev) if st["ad"] else None
         losses = torch.empty(4, device=dev)
         waves = _lib.lib().sn_render_step_waves(st["B"], P)
         partials = torch.empty(4 * waves, device=dev)

@@ -159,17 +159,29 @@ class Runner:
     def occ_eval_fn(self):
         return self.fused_step.occ_eval_fn if self.fused else self.renderer.occ_eval_fn
 
-    def train_iteration(self, iter_i, ahead=None):
-        """One iteration; returns the loss tensor (or None when skipped).
-        In graph mode the next iteration's march may run ahead (see
-        _march_ahead_step); `ahead` False forbids it, True allows it past
-        end_iter (a caller that will run more iterations)."""
+    def begin_iteration(self, iter_i):
+        """The host part of an iteration before its step (exp_runner.py:147-159):
+        step size, occupancy update when due, bandwidth; returns the step size."""
         r = self.renderer
         r.sampling_step_size = 10 ** (np.log10(self.start_step_size) - self.slop_step * iter_i)
         r.occupancy_grid.every_n_step(step=iter_i, occ_eval_fn=self.occ_eval_fn(),
                                       occ_thre=self.occ_threshold, n=self.occ_update_freq)
         if self.iter_step % self.increase_bindwidth_every == 0:
             self.sdf_network.increase_bandwidth()
+        return r.sampling_step_size
+
+    def end_iteration(self):
+        """After a device iteration that another driver replayed."""
+        self.iter_step += 1
+        self.update_learning_rate()
+
+    def train_iteration(self, iter_i, ahead=None):
+        """One iteration; returns the loss tensor (or None when skipped).
+        In graph mode the next iteration's march may run ahead (see
+        _march_ahead_step); `ahead` False forbids it, True allows it past
+        end_iter (a caller that will run more iterations)."""
+        r = self.renderer
+        self.begin_iteration(iter_i)
         if self.fused and self.sf_step is not None:
             graph = self.step_mode == "graph"
             losses = self.sf_step.step(self.lr, r.sampling_step_size, use_graph=graph,

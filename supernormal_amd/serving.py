@@ -12,6 +12,15 @@ Chamfer distance to the ground-truth visible points falls under the job's
 tolerance (checked every `eval_every` iterations) or at `max_iters`; then its
 mesh is extracted and its memory released.
 
+Shared launches (the default, `fused=True`): after a job's first iteration
+(its own graphs, which also warm its kernels up), the running jobs' iterations
+are captured together into ONE HIP graph per set of jobs -- each job's
+sampling, march, rendering, backward and step guard on a stream forked from
+the capture stream, then a single Adam launch over every job's parameters
+(sn_adam_multi, one segment per job) -- and a round is one replay of it.  The
+graph is captured again when a job joins or leaves.  With `fused=False` each
+job replays its own graphs on its own stream.
+
 Each job draws its patches and occupancy samples from a generator of its own,
 so its result does not depend on what else runs beside it (up to the order
 of fp32 atomics in the hash-grid backward).
@@ -44,7 +53,7 @@ class JobSpec:
     max_iters: int = 2000
     chamfer_tol: float = 5e-3
     eval_every: int = 250
-    mesh_resolution: int = 128
+    mesh_resolution: int = 256
 
 
 def sample_stream(n_jobs=64, rate=0.5, seed=0, views=(8, 50), resolutions=(256, 512, 768, 1024),
@@ -115,6 +124,7 @@ class Job:
         self.iters = 0
         self.samples = torch.zeros(1, dtype=torch.float64, device=self.device)
         self.chamfer = None
+        self.mesh_dev = None
         self.history = []
         with torch.cuda.stream(self.stream):
             if data is None:
@@ -131,6 +141,20 @@ class Job:
         self.ready = torch.cuda.Event()
         self.ready.record(self.stream)
 
+    def begin_joint(self):
+        """Host part of the job's next iteration inside a joint round (on the
+        current stream); returns the step count token for end_joint()."""
+        r = self.runner
+        step = r.begin_iteration(self.iters)
+        return r.sf_step.prepare(r.lr, step)
+
+    def end_joint(self, t):
+        r = self.runner
+        r.sf_step.finished(t)
+        r.end_iteration()
+        self.samples += r.sf_step.ns
+        self.iters += 1
+
     def advance(self, n):
         """Issue n iterations on the job's stream (asynchronous)."""
         with torch.cuda.stream(self.stream):
@@ -146,13 +170,27 @@ class Job:
             self.iters > 0 and self.iters % self.spec.eval_every == 0)
 
     def evaluate(self):
-        """Chamfer of the learned surface's visible points against the ground truth."""
-        from .mesh import chamfer_distance_and_f1_score, visible_surface_points
+        """Chamfer of the learned surface against the ground truth, the
+        reference's way (exp_runner.py:579-630): marching cubes of the SDF at
+        the job's mesh resolution, first hits of training-pixel rays with the
+        mesh; the mesh is kept as the job's result."""
+        from .mesh import (chamfer_distance_and_f1_score, extract_fields_device,
+                           marching_cubes_device, visible_mesh_points)
+        # after the iterations issued on the current stream (joint rounds)
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(self.stream):
             net = self.runner.sdf_network
-            pts = visible_surface_points(lambda x: net.fused_sdf(x), self.runner.dataset,
-                                         max(1, 20000 // len(self.runner.dataset.train_images)))
+            r = self.spec.mesh_resolution
+            lo, hi = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+            u = extract_fields_device(lo, hi, r, lambda x: -net.fused_sdf(x.contiguous()),
+                                      self.device)
+            self.mesh_dev = marching_cubes_device(-u, 0.0, lo, hi)
+            del u
+            pts = visible_mesh_points(self.mesh_dev, self.runner.dataset,
+                                      max(1, 20000 // len(self.runner.dataset.train_images)))
             cd, fs = chamfer_distance_and_f1_score(self.gt, pts, f_threshold=0.005)
+        # later iterations (next rounds) overwrite what the evaluation read
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
         self.chamfer = cd
         self.history.append((self.iters, cd, fs))
         return cd
@@ -162,13 +200,10 @@ class Job:
             self.chamfer is not None and self.chamfer <= self.spec.chamfer_tol)
 
     def extract_mesh(self):
-        from .mesh import extract_geometry
+        """The mesh of the last evaluation (a job finishes right after one) as
+        PyMCubes returns it: vertices, triangles."""
         with torch.cuda.stream(self.stream):
-            net = self.runner.sdf_network
-            r = self.spec.mesh_resolution
-            v, f = extract_geometry(np.array([-1.0, -1.0, -1.0]), np.array([1.0, 1.0, 1.0]), r,
-                                    0.0, lambda x: -net.fused_sdf(x.contiguous()))
-        return v, f
+            return self.mesh_dev.indexed()
 
     def finish(self):
         """Mesh extraction; returns when the job's work is done."""
@@ -180,15 +215,72 @@ class Job:
     def release(self):
         self.runner = None
         self.gt = None
+        self.mesh_dev = None
+
+
+class JointRound:
+    """One HIP graph holding one iteration of every job in `jobs`: each job's
+    iteration up to its step guard on a stream of its own forked from the
+    capture stream, joined, then one sn_adam_multi launch over all the jobs'
+    parameters.  replay() runs one iteration of all of them."""
+
+    def __init__(self, jobs, device):
+        import ctypes as C
+        from . import _lib
+        self.jobs = list(jobs)
+        self.device = torch.device(device)
+        lib = _lib.lib()
+        segs = (_lib.SnAdamSegment * len(self.jobs))()
+        block0 = 0
+        for i, job in enumerate(self.jobs):
+            sf = job.runner.sf_step
+            nb = int(lib.sn_adam_multi_blocks(sf.flat.n))
+            segs[i] = _lib.SnAdamSegment(*sf.adam_segment(), block0, nb)
+            block0 += nb
+        self.total_blocks = block0
+        raw = bytes(bytearray(segs))
+        self.segments = torch.tensor(list(raw), dtype=torch.uint8).to(self.device)
+        self.graph = self._capture()
+
+    def _capture(self):
+        from ._lib import call, ptr, stream_of
+        dev = self.device
+        torch.cuda.synchronize(dev)
+        g = torch.cuda.CUDAGraph()
+        for job in self.jobs:
+            gen = job.runner.sf_step.gen
+            if gen is not None:
+                g.register_generator_state(gen)
+        streams = [torch.cuda.Stream(device=dev) for _ in self.jobs]
+        with torch.cuda.graph(g):
+            cur = torch.cuda.current_stream(dev)
+            for job, st in zip(self.jobs, streams):
+                st.wait_stream(cur)
+                with torch.cuda.stream(st):
+                    job.runner.sf_step.capture_parts()
+            for st in streams:
+                cur.wait_stream(st)
+            call("sn_adam_multi", ptr(self.segments), len(self.jobs), self.total_blocks, 0.9,
+                 0.999, 1e-8, stream_of(dev))
+        self._streams = streams
+        return g
+
+    def replay(self):
+        tokens = [job.begin_joint() for job in self.jobs]
+        self.graph.replay()
+        for job, t in zip(self.jobs, tokens):
+            job.end_joint(t)
 
 
 class Server:
-    """Admission by HBM budget and slot count; round-robin issue of graph
-    replays over the running jobs' streams."""
+    """Admission by HBM budget and slot count; each round advances every
+    running job by `chunk` iterations: replays of one joint graph over the
+    running jobs (fused=True), or each job's own graphs on its own stream."""
 
     def __init__(self, device="cuda", hbm_budget_bytes=None, max_running=8, chunk=10,
                  precision="fp16", make_job: Optional[Callable] = None,
-                 clock: Callable[[], float] = time.perf_counter, sleep=time.sleep):
+                 clock: Callable[[], float] = time.perf_counter, sleep=time.sleep,
+                 fused=True):
         self.device = torch.device(device)
         if hbm_budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -199,6 +291,37 @@ class Server:
         self.precision = precision
         self.make_job = make_job or (lambda spec: Job(spec, self.device, precision))
         self.clock, self.sleep = clock, sleep
+        self.fused = bool(fused)
+        self.captures = 0
+        self._joint = None
+        self._joint_key = None
+
+    def _issue(self, running):
+        """One round: `chunk` iterations of every running job."""
+        joint = [j for j in running if self.fused and hasattr(j, "begin_joint")]
+        for job in running:
+            if job not in joint:
+                job.advance(self.chunk)
+        # a job's first iteration runs on its own graphs (no march left pending)
+        for job in joint:
+            if job.iters == 0:
+                with torch.cuda.stream(job.stream):
+                    job.runner.train_iteration(0, ahead=False)
+                    job.samples += job.runner.sf_step.ns
+                    job.iters += 1
+                torch.cuda.current_stream(job.device).wait_stream(job.stream)
+        members = [j for j in joint if _joint_ready(j)]
+        if not members:
+            return
+        key = tuple(id(j) for j in members)
+        if self._joint_key != key:
+            self._joint = JointRound(members, members[0].device)
+            self._joint_key = key
+            self.captures += 1
+        for _ in range(self.chunk):
+            self._joint.replay()
+            if any(j.due_for_eval() for j in members):
+                break
 
     def run(self, specs: List[JobSpec], log=None, on_done=None):
         """Serve `specs`; `on_done(job)` sees each finished job before its state
@@ -229,8 +352,7 @@ class Server:
                     self.sleep(max(0.0, queue[0].arrival_s - (self.clock() - t0)))
                 continue
             tp = self.clock()
-            for job in running:
-                job.advance(self.chunk)
+            self._issue(running)
             phase["issue"] += self.clock() - tp
             done = []
             for job in running:
@@ -260,11 +382,20 @@ class Server:
                 job.release()
         t_end = self.clock() - t0
         ttm = np.array([r["time_to_mesh_s"] for r in records])
+        self._joint = None
+        self._joint_key = None
         return dict(records=records, p50_s=float(np.percentile(ttm, 50)),
                     p99_s=float(np.percentile(ttm, 99)), makespan_s=t_end,
+                    joint_captures=self.captures, fused=self.fused,
                     samples_per_s=float(sum(r["samples"] for r in records)) / max(t_end, 1e-9),
                     peak_running=peak_running,
                     host_s={k: round(v, 3) for k, v in phase.items()})
+
+
+def _joint_ready(job):
+    """A job takes part in joint rounds once its own first iteration ran (its
+    kernels warmed up, nothing pending) and while iterations remain."""
+    return job.iters > 0 and job.iters < job.spec.max_iters
 
 
 def run_stream(specs, device="cuda", one_at_a_time=False, **kw):
@@ -287,6 +418,8 @@ def main(argv=None):
     ap.add_argument("--resolutions", default="256,512,768,1024")
     ap.add_argument("--views", default="8,50")
     ap.add_argument("--one-at-a-time", action="store_true")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="each job replays its own graphs on its own stream")
     a = ap.parse_args(argv)
     specs = sample_stream(a.jobs, a.rate, a.seed,
                           views=tuple(int(v) for v in a.views.split(",")),
@@ -304,7 +437,7 @@ def main(argv=None):
         print(json.dumps({"input_ready": sp.job_id, "s": round(time.perf_counter() - t, 2)}),
               flush=True)
     max_running = 1 if a.one_at_a_time else a.max_running
-    server = Server(max_running=max_running, precision="fp16",
+    server = Server(max_running=max_running, precision="fp16", fused=not a.no_fuse,
                     make_job=lambda sp: Job(sp, "cuda", "fp16", data=data.pop(sp.job_id)))
     out = server.run(specs, log=log)
     out.pop("records")

@@ -495,17 +495,23 @@ class SyncFreeStep:
                  *self._hash_args(), ptr(self.levels_dev), ptr(self.g_enc_dot),
                  ptr(self.g_grad), None, ptr(params.grad), None, s)
 
-    def _part_adam(self):
-        """Adam with the learning rate and step count on the device."""
+    def _part_guard(self):
+        """Whether this iteration's optimiser step runs, and the device step
+        count.  An iteration with no visible sample is skipped (parameters,
+        moments and step count unchanged; the reference raises KeyError there,
+        exp_runner.py:182) and counted in n_skipped, and so is one whose losses
+        or gradients are not finite (also counted in n_nonfinite)."""
         s = stream_of(self.dev)
-        # an iteration with no visible sample is skipped (parameters, moments and
-        # step count unchanged; the reference raises KeyError there,
-        # exp_runner.py:182) and counted in n_skipped
         f = self.flat
-        # ... and so is one whose losses or gradients are not finite
         call("sn_step_guard", ptr(self.ns), ptr(self.losses), 3, ptr(f.grad), f.n,
              ptr(self.run), ptr(self.n_nonfinite), s)
         call("sn_step_advance", ptr(self.adam_t), ptr(self.run), ptr(self.n_skipped), s)
+
+    def _part_adam(self):
+        """Adam with the learning rate and step count on the device."""
+        s = stream_of(self.dev)
+        self._part_guard()
+        f = self.flat
         t16 = self.table16
         call("sn_adam_step_copy16", ptr(f.data), ptr(f.grad), ptr(f.exp_avg),
              ptr(f.exp_avg_sq), f.n, 0.0, ptr(self.lr), 0.9, 0.999, 1e-8, 0,
@@ -587,6 +593,51 @@ class SyncFreeStep:
         self.flat.step_count += 1
         self._adam_t_host = t + 1
         return self.losses
+
+    # ------------------------------------------- driven by another graph owner
+    def prepare(self, lr, step_size):
+        """Writes this iteration's device scalars (learning rate, step size,
+        active levels, the optimiser's step count if the host changed it) for
+        an iteration that another driver replays (the serving loop's joint
+        graph, which holds capture_parts() and an Adam launch); returns the
+        host step count to hand to finished()."""
+        if self._pending is not None:
+            raise RuntimeError("a march is pending; drain() it first")
+        self.lr.fill_(float(lr))
+        levels = int(self.net.active_levels())
+        if levels != self._levels_host:
+            self.levels_dev.fill_(levels)
+            self._levels_host = levels
+        t = int(self.flat.step_count)
+        if t != self._adam_t_host:
+            self.adam_t.fill_(t)
+        self._adam_t_host = None
+        self.step_size.fill_(float(step_size))
+        return t
+
+    def finished(self, t):
+        """After the replay of an iteration prepared with prepare()."""
+        self.flat.step_count = t + 1
+        self._adam_t_host = t + 1
+
+    def capture_parts(self):
+        """The iteration up to the optimiser step (sampling, march, rendering,
+        backward, step guard), enqueued on the current stream, for a graph that
+        another driver captures; the Adam step is the caller's
+        (adam_segment())."""
+        self._part_march()
+        self._part_rest()
+        self._part_hash()
+        self._part_guard()
+
+    def adam_segment(self):
+        """This optimiser's row of sn_adam_multi's segment table, without the
+        block range: (params, grads, exp_avg, exp_avg_sq, n, lr, step, run,
+        fp16 copy, copy offset, copy length)."""
+        f, t16 = self.flat, self.table16
+        return (ptr(f.data), ptr(f.grad), ptr(f.exp_avg), ptr(f.exp_avg_sq), f.n, ptr(self.lr),
+                ptr(self.adam_t), ptr(self.run), ptr(t16),
+                self.table_off if t16 is not None else 0, t16.numel() if t16 is not None else 0)
 
     def _step_given(self, lr, step_size, inputs):
         """Graph mode on given inputs: they are copied into the static buffers,
@@ -723,22 +774,28 @@ class SyncFreeStep:
 
     def _capture(self):
         # one eager warm-up on a side stream (allocator pools, kernel attributes),
-        # then capture; the warm-up's update is undone from saved state.  A march
+        # then capture; the warm-up's update (parameters, moments, step count,
+        # fp16 table copy, counters) is undone from saved state.  A march
         # that ran ahead keeps its inputs: the warm-up re-marches them ("keep").
         f = self.flat
         torch.cuda.synchronize(self.dev)
-        saved = [t.clone() for t in (f.data, f.exp_avg, f.exp_avg_sq, self.adam_t)]
-        gen_state = self.gen.get_state() if self.gen is not None else None
+        kept = (f.data, f.exp_avg, f.exp_avg_sq, self.adam_t, self.stat, self.stat_acc)
+        saved = [t.clone() for t in kept]
+        gen_state = self._rng_state()
         side = torch.cuda.Stream(device=self.dev)
         side.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(side):
             self._iteration("keep" if self._pending is not None else None)
         torch.cuda.current_stream(self.dev).wait_stream(side)
         torch.cuda.synchronize(self.dev)
-        for t, v in zip((f.data, f.exp_avg, f.exp_avg_sq, self.adam_t), saved):
+        for t, v in zip(kept, saved):
             t.copy_(v)
-        if gen_state is not None:
+        # the warm-up's Adam step also rewrote the fp16 table copy
+        self.refresh_copies()
+        if self.gen is not None:
             self.gen.set_state(gen_state)
+        else:
+            torch.cuda.set_rng_state(gen_state, self.dev)
         graphs = [torch.cuda.CUDAGraph() for _ in range(4)]
         if self.gen is not None:
             graphs[0].register_generator_state(self.gen)

@@ -13,11 +13,15 @@ patches (64 patches of 3x3 rays) at the full-size workload's field (hash grid
     renderer.py:142-276 and fields.py:76-119), with the device's march and
     visibility decision, for the 6 entries of largest gradient of each
     parameter tensor (hash table, each layer's weight_v / weight_g / bias,
-    the variance).
+    the variance); 3 for 'ad'.
 
 Tolerance: rtol 1e-3 (plus atol 1e-5 of the tensor's largest checked
-entry).  The oracle's own fp64 autograd gradient must agree with the finite
-differences to rtol 1e-6 (which pins the oracle's backward as well).
+entry), widened by three times the error of the same oracle evaluated in
+float32 on the CPU where fp32 itself cannot reach 1e-3: with 'ad' the
+first layer's weight gradient sums second-derivative terms of the softplus
+(beta = 100: softplus'' up to 25) that cancel across the samples.  The
+oracle's own fp64 autograd gradient must agree with the finite differences
+to rtol 1e-6 (which pins the oracle's backward as well).
 """
 import numpy as np
 import pytest
@@ -26,7 +30,9 @@ import torch
 pytestmark = pytest.mark.gpu
 
 N_PATCHES = 64
-PER_TENSOR = 6
+# entries checked per parameter tensor ('ad' costs the oracle a double
+# backward through the fp64 hash encoding per loss evaluation)
+PER_TENSOR = {"dfd": 6, "ad": 3}
 
 
 def _runner(dev, mode, method):
@@ -107,6 +113,17 @@ def test_kernel_gradients_vs_fp64_finite_differences(dev, method):
     print(f"\n[{method}] visible samples {res['ri'].size}, loss {res['loss']:.6e}", flush=True)
     go = {k: v.grad.detach().double().reshape(-1).clone() for k, v in _oracle_tensors(tr).items()}
     tensors = _oracle_tensors(tr)
+    # the same oracle in float32: what fp32 arithmetic itself reaches
+    tr32 = OracleTrainer(r.sdf_network, float(r.deviation_network.variance.detach()),
+                         r.occ_resolution, {"lr": r.lr, "loss_type": r.loss_type},
+                         dtype=torch.float32)
+    tr32.binary = tr.binary
+    with torch.no_grad():
+        for (k, a), b in zip(tensors.items(), _oracle_tensors(tr32).values()):
+            b.copy_(a.detach().float())
+    assert tr32.forward_backward(*cpu, step, **kw) is not None
+    g32 = {k: v.grad.detach().double().reshape(-1).clone()
+           for k, v in _oracle_tensors(tr32).items()}
 
     def loss():
         with torch.enable_grad():
@@ -115,7 +132,7 @@ def test_kernel_gradients_vs_fp64_finite_differences(dev, method):
     h = 1e-6
     for name, t in tensors.items():
         g_o = go[name]
-        idx = torch.argsort(g_o.abs(), descending=True)[:PER_TENSOR]
+        idx = torch.argsort(g_o.abs(), descending=True)[:PER_TENSOR[method]]
         fd = []
         for i in idx.tolist():
             flat = t.data.view(-1)
@@ -129,9 +146,13 @@ def test_kernel_gradients_vs_fp64_finite_differences(dev, method):
         fd = np.array(fd)
         dev_g = gd[name][idx].numpy()
         orc_g = g_o[idx].numpy()
+        o32 = g32[name][idx].numpy()
         scale = np.abs(fd).max()
         rel = np.abs(dev_g - fd) / np.maximum(np.abs(fd), 1e-300)
+        rel32 = np.abs(o32 - fd) / np.maximum(np.abs(fd), 1e-300)
         print(f"[{method}] {name:16s} max|fd| {scale:.3e}  device rel err max {rel.max():.2e}  "
-              f"oracle rel err max {(np.abs(orc_g - fd) / np.abs(fd)).max():.2e}", flush=True)
+              f"fp32 oracle {rel32.max():.2e}  "
+              f"fp64 oracle {(np.abs(orc_g - fd) / np.abs(fd)).max():.2e}", flush=True)
         np.testing.assert_allclose(orc_g, fd, rtol=1e-6, atol=1e-9 * scale)
-        np.testing.assert_allclose(dev_g, fd, rtol=1e-3, atol=1e-5 * scale)
+        bound = 1e-3 * np.abs(fd) + 3.0 * np.abs(o32 - fd) + 1e-5 * scale
+        assert np.all(np.abs(dev_g - fd) <= bound), (name, rel, rel32)

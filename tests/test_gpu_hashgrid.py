@@ -309,3 +309,4 @@ def test_device_level_count(dev, k):
         outs.append((enc, gp, gx))
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+

@@ -1,0 +1,167 @@
+"""The large single-object workload's shapes on the GPU (hash table 2^21 with
+16 levels x 2 features, occupancy grid 256^3, 8192 patches of 3x3 rays, up to
+256 samples per ray, 50 views at 1024^2):
+
+  * hash-grid forward / backward (bucket records + fixed-point sums) at
+    T = 2^21 against the fp64 oracle on 4096 points;
+  * the same backward over 8192 x 9 x 256 = 18.9 M points (a full iteration's
+    patch samples at the sample cap), checked by what the interpolation fixes
+    whatever the size: the trilinear weights of a point sum to one, so for
+    every level and feature the table gradient's entries sum to the sum of
+    dL/denc over the points (a checksum of the scatter, summed in fp64 on
+    the host), and masked levels receive nothing;
+  * ray marching on a 256^3 grid bit-exact against the C oracle (8192 rays);
+  * one iteration of the prims1024 workload on the device iteration
+    (SyncFreeStep, fp32, no fp16 table copy) against FusedStep on the same
+    8192 patches: the same marched / visible counts, loss and gradients within
+    the order of fp32 atomics.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle
+from oracle.hashgrid_oracle import encode, hashgrid_desc
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(n_levels=16, n_features_per_level=2, log2_hashmap_size=21, base_resolution=16,
+           per_level_scale=1.38)
+
+
+def _enc(dev, seed=0):
+    from supernormal_amd.tcnn import Encoding
+    e = Encoding(3, {"otype": "HashGrid", **CFG}).to(dev)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    with torch.no_grad():
+        e.params.copy_((torch.rand(e.params.shape, generator=g) * 2 - 1).to(dev))
+    return e
+
+
+def test_hash_t21_vs_fp64(dev):
+    e = _enc(dev, 1)
+    d = hashgrid_desc(16, 21, 16, 1.38)
+    assert e.params.numel() == 2 * d["n_entries"] == 42208800
+    rng = np.random.default_rng(3)
+    x = (rng.random((4096, 3)) * 2 - 1).astype(np.float32)
+    xt = torch.from_numpy(x).to(dev)
+    out = e(xt)
+    g = torch.from_numpy(rng.normal(size=out.shape).astype(np.float32)).to(dev)
+    (out * g).sum().backward()
+    p64 = e.params.detach().cpu().double().requires_grad_(True)
+    ref = encode(torch.from_numpy(x).double(), p64, d)
+    (ref * g.cpu().double()).sum().backward()
+    torch.testing.assert_close(out.detach().cpu().double(), ref.detach(), rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(e.params.grad.cpu().double(), p64.grad, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("n_active", [16, 11])
+def test_hash_backward_checksum_at_full_size(dev, n_active):
+    from supernormal_amd.tcnn import _gargs, bucket_workspace_bytes, hashgrid_bwd
+    from supernormal_amd._lib import stream_of
+    e = _enc(dev, 2)
+    grid = e.grid
+    n = 8192 * 9 * 256
+    g = torch.Generator(device=dev).manual_seed(4)
+    # points clustered like patch samples: 8192 rays' worth of short segments
+    c = torch.rand(8192, 1, 3, device=dev, generator=g) * 1.8 - 0.9
+    t = torch.linspace(0, 0.2, 9 * 256, device=dev)[None, :, None]
+    dirs = torch.nn.functional.normalize(torch.randn(8192, 1, 3, device=dev, generator=g), dim=-1)
+    x = (c + dirs * t).reshape(-1, 3).clamp(-1, 1).contiguous()
+    assert x.shape[0] == n
+    genc = torch.randn(n, 32, device=dev, generator=g)
+    gp = torch.zeros_like(e.params)
+    gargs = _gargs(grid, n_active)
+    ws = torch.empty(bucket_workspace_bytes(gargs, n), dtype=torch.uint8, device=dev)
+    print(f"\n{n} points, {n_active} active levels, workspace {ws.numel() / 2**30:.2f} GiB",
+          flush=True)
+    hashgrid_bwd(x, n, None, e.params, gargs, genc, gp, None, stream_of(dev), ws=ws)
+    torch.cuda.synchronize()
+    offs = e.level_offsets()
+    gp_h = gp.cpu().double().numpy().reshape(-1, 2)
+    ge_h = genc.cpu().double().numpy().reshape(n, 16, 2)
+    for l in range(16):
+        s = gp_h[offs[l]:offs[l + 1]].sum(0)
+        if l < n_active:
+            want = ge_h[:, l].sum(0)
+            scale = np.abs(ge_h[:, l]).sum(0)
+            assert np.all(np.abs(s - want) <= 1e-5 * scale), (l, s, want)
+        else:
+            assert np.all(gp_h[offs[l]:offs[l + 1]] == 0.0), l
+
+
+def _grid256():
+    c = (np.arange(256) + 0.5) / 256 * 2 - 1
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    ridge = 0.02 * np.sin(40 * X) * np.sin(40 * Y)
+    return ((r < 0.6 + ridge) & (r > 0.3)).astype(np.uint8)
+
+
+def test_ray_marching_256_bitexact(dev):
+    from supernormal_amd.nerfacc import march_raw
+    rng = np.random.default_rng(12)
+    n = 8192
+    o = np.repeat(np.array([[0.2, -0.3, -2.5]], np.float32), n, 0)
+    d = rng.normal(size=(n, 3)) * 0.15 + np.array([-0.08, 0.12, 1.0])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    b = 2 * (o * d).sum(1)
+    cc = (o ** 2).sum(1) - 1
+    disc = np.maximum(b * b - 4 * cc, 0)
+    near = ((-b - np.sqrt(disc)) / 2).astype(np.float32)
+    far = ((-b + np.sqrt(disc)) / 2).astype(np.float32)
+    step = np.float32(1e-3)
+    near = near + rng.random(n).astype(np.float32) * step
+    grid = _grid256()
+    roi = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    pi, ri, t0, t1 = march_raw(t(o), t(d), t(near), t(far), t(roi), t(grid.astype(bool)),
+                               float(step), 0.0, 256)
+    pi_o, ri_o, t0_o, t1_o = c_oracle.ray_marching(o, d, near, far, roi, grid, float(step),
+                                                   max_per_ray=256)
+    print(f"\nmarched samples {ri_o.size}", flush=True)
+    assert ri_o.size > 100000
+    np.testing.assert_array_equal(pi.cpu().numpy(), pi_o)
+    np.testing.assert_array_equal(ri.cpu().numpy(), ri_o)
+    np.testing.assert_array_equal(t0.cpu().numpy(), t0_o)
+    np.testing.assert_array_equal(t1.cpu().numpy(), t1_o)
+
+
+def test_prims1024_iteration_matches_fused_step(dev):
+    from supernormal_amd.runner import Runner, synthetic_conf
+    from supernormal_amd.step import SyncFreeStep
+    conf = synthetic_conf("prims1024", mlp_precision="fp32", step_mode="fused",
+                          max_samples_per_ray=256)
+    r = Runner(conf, fused=True, device=dev, seed=5)
+    r.update_learning_rate()
+    for i in range(40):
+        r.train_iteration(i)
+    while r.sdf_network.active_levels() < 16:
+        r.sdf_network.increase_bandwidth()
+    torch.cuda.synchronize()
+    fs, flat = r.fused_step, r.optimizer
+    sf = SyncFreeStep(r.sdf_network, r.deviation_network, r.renderer.occupancy_grid, flat,
+                      r.dataset, r.batch_size, r.patch_size, r.max_samples_per_ray,
+                      r.normal_weight, r.eikonal_weight, r.mask_weight, r.loss_type,
+                      precision="fp32", device=dev, half_table=False)
+    g = torch.Generator(device=dev).manual_seed(17)
+    o, d, pn, vi, nt, m = r.dataset.gen_random_patches(r.batch_size, generator=g)
+    assert o.shape[0] == 8192 and r.renderer.occupancy_grid.binary.shape[0] == 256
+    near, far = r.dataset.near_far_from_sphere(o[:, 1, 1], d[:, 1, 1])
+    step = 4e-3
+    flat.zero_grad()
+    res = fs.forward_backward(o, d, pn, vi, nt, m, near, far, step, stratified=False)
+    flat.collect()
+    g_ref = flat.grad.clone()
+    loss_ref = float(res.loss)
+    losses = sf.step(r.lr, step, use_graph=False, inputs=(o, d, pn, vi, nt, m))
+    torch.cuda.synchronize()
+    print(f"\nmarched {res.n_marched}, visible {res.n_samples}, loss {loss_ref:.6e}", flush=True)
+    assert int(sf.n0) == res.n_marched and int(sf.ns) == res.n_samples
+    loss = float(losses[0] + losses[1] + losses[2])
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    scale = float(g_ref.abs().max())
+    # the MLP weights are packed by different kernels in the two paths (one-ulp
+    # differences of the weight-norm sums), so the bound is looser than the
+    # fixed-order sums alone would need
+    torch.testing.assert_close(flat.grad, g_ref, rtol=1e-3, atol=1e-4 * scale)

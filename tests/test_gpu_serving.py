@@ -1,5 +1,6 @@
 """Serving on the GPU: three small jobs run alone and together in the request
-loop give the same results within tolerance.
+loop give the same results within tolerance, with shared launches (one joint
+graph per round, one Adam launch for all jobs) and with per-job graphs.
 
 Each job samples from generators of its own, so running beside other jobs
 changes nothing but the order of fp32 atomics in the hash-grid backward; Adam
@@ -23,7 +24,7 @@ def _specs():
             for i in range(3)]
 
 
-def _run(dev, specs, max_running):
+def _run(dev, specs, max_running, fused=True):
     g = torch.Generator(device="cpu").manual_seed(0)
     pts = (torch.rand(4096, 3, generator=g) * 1.6 - 0.8).to(dev)
     fields = {}
@@ -34,18 +35,24 @@ def _run(dev, specs, max_running):
                                        [h[1] for h in job.history])
 
     srv = serving.Server(device=dev, hbm_budget_bytes=64 << 30, max_running=max_running,
-                         chunk=5)
+                         chunk=5, fused=fused)
     out = srv.run(specs, on_done=keep)
     return fields, out
 
 
-def test_jobs_same_alone_and_in_the_mix(dev):
+@pytest.mark.parametrize("fused", [True, False])
+def test_jobs_same_alone_and_in_the_mix(dev, fused):
+    """fused: the jobs' iterations replayed as one joint graph per round with
+    one Adam launch over all of them; else each job's own graphs on its own
+    stream.  Alone = one job at a time in the same mode."""
     alone = {}
     for s in _specs():
-        f, _ = _run(dev, [s], 1)
+        f, _ = _run(dev, [s], 1, fused)
         alone.update(f)
-    mixed, out = _run(dev, _specs(), 3)
+    mixed, out = _run(dev, _specs(), 3, fused)
     assert out["peak_running"] == 3
+    if fused:
+        assert out["joint_captures"] >= 1
     assert len(out["records"]) == 3
     for j in alone:
         sdf_a, cd_a = alone[j]
