@@ -368,3 +368,21 @@ def test_hashgrid_grad_fd():
 
     # finite differences stay inside one cell for tiny eps
     assert torch.autograd.gradcheck(f, (x, p), eps=1e-7, atol=1e-5, rtol=1e-4)
+
+
+# ---------------------------------------------------------------- sanitizers
+def test_oracle_under_address_and_ub_sanitizers():
+    """The C oracle built with -fsanitize=address,undefined runs every
+    function on exact-size heap buffers (empty rays, the sample cap,
+    out-of-range indices, NaN occupancies, axis-aligned rays) without a
+    report; any report aborts the driver."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+    if not shutil.which("make") or not shutil.which("gcc"):
+        pytest.skip("make / gcc not present")
+    root = Path(__file__).resolve().parents[1] / "oracle"
+    r = subprocess.run(["make", "-s", "-C", str(root), "sanitize"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "all functions ran clean" in r.stdout
