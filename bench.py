@@ -36,6 +36,9 @@ the same run.  Beside it:
     late launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs over
     --traffic-probe, per point, times the timed launches' mean points;
   * fp32: the same schedule with the fp32 MLP and fp32 table (--no-fp32 skips);
+  * method_ad: the same schedule with the analytic gradient (method ad: the
+    rendered normal is the SDF's x-gradient, trained through its double
+    backward) (--no-ad skips);
   * accuracy: Chamfer distance / F-score of the trained field's visible
     surface points against the analytic scene's, for each leg;
   * plain_gpu: the reference-structured path (torch autograd over the drop-in
@@ -500,6 +503,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-small", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-ad", action="store_true",
+                    help="skip the leg that trains the same schedule with the analytic "
+                         "gradient (method ad)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-gate", action="store_true",
                     help="skip the small workload's run against the FP64 oracle")
@@ -581,6 +587,20 @@ def main(argv=None):
             out["fp32"]["accuracy"] = accuracy(r32)
         log(f"fp32 leg: {res32['value']:.1f} it/s")
         del r32
+        torch.cuda.empty_cache()
+    if not a.no_ad and a.method != "ad":
+        rad, resad = run_schedule(dev, a.workload, a.precision, "ad", mode, a.seed, a.warmup,
+                                  a.steps, a.max_samples, log=log)
+        out["method_ad"] = {"value": round(resad["value"], 2), "unit": "train iters/s",
+                            "ms_per_step": round(resad["ms_per_step"], 4),
+                            "full_schedule": round(resad["full_schedule"], 2),
+                            "note": "same schedule and seed, rendered normal = analytic "
+                                    "x-gradient of the SDF (renderer.py:225-226), double "
+                                    "backward on the fp32 MLP kernels"}
+        if not a.no_accuracy:
+            out["method_ad"]["accuracy"] = accuracy(rad)
+        log(f"method ad leg: {resad['value']:.1f} it/s")
+        del rad
         torch.cuda.empty_cache()
     if a.plain_full:
         pf = plain_full(dev, a.workload, a.seed, a.max_samples, log=log)

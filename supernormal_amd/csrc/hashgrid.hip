@@ -410,11 +410,13 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    The lists share one pool; bucket_layout_kernel (before the records)
 //    gives each list 5/4 of the records it took in the previous call on the
 //    same workspace plus a floor (a quarter of the pool split evenly), at
-//    least the even split and at most kMaxList (or the even split if
-//    larger), scaled down to the pool (the
-//    even split when the workspace holds no previous call's layout): with a
-//    pool smaller than kMaxList per bucket, the buckets that take many
-//    records get the room.
+//    most kMaxList (or the even split if larger); then either scales these
+//    down to the pool or shares out what they leave of it evenly (up to the
+//    same cap), and uses the even split when the workspace holds no previous
+//    call's layout.  So the buckets that take many records get the room, and
+//    a pool of at least 5/4 of a call's records holds every list of a call
+//    that repeats the previous one's counts: each list's share is then at
+//    least (5/4 c) * pool / (5/4 records + pool / 4) >= c.
 //    The largest |value| of each level is kept (integer atomic max on the
 //    bits of a non-negative float).
 // 2. hash_bwd_bucket_sum_kernel: one workgroup per bucket sums its records in
@@ -500,7 +502,7 @@ bucket_layout_kernel(BucketPool pool, int32_t nb) {
     if (q < per && b < nb) {
       if (valid) {
         const int64_t prev = min((int64_t)max(pool.cursor[b], 0), total);
-        w = max(even, prev + prev / 4 + floor_share);
+        w = prev + prev / 4 + floor_share;
       } else {
         w = even;
       }
@@ -517,11 +519,19 @@ bucket_layout_kernel(BucketPool pool, int32_t nb) {
 #pragma unroll
   for (int k = 0; k < kWaves; ++k) tot += wsum[k];
   __syncthreads();
-  // scale down to the pool (w <= 2.25 total < 2^31.2, total <= 2^30: no overflow)
+  // scale down to the pool, by the 32.32 fixed-point factor total / tot
+  // (w <= 2.25 total < 2^31.2, total <= 2^30: w * factor < 2^64; the
+  // rounding keeps the sum within the pool), or share what the wants leave
+  // of it evenly (up to the cap)
+  const int64_t cap = max(even, kMaxList);
+  const bool scale = tot > total;
+  const uint64_t factor = scale ? ((uint64_t)total << 32) / (uint64_t)tot : 0;
+  const int64_t extra = scale ? 0 : (total - tot) / nb;
   long long mine = 0;
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
-    if (tot > total) want[q] = want[q] * total / tot;
+    if (scale) want[q] = (int64_t)(((uint64_t)want[q] * factor) >> 32);
+    else want[q] = want[q] > 0 ? min(want[q] + extra, cap) : 0;  // want <= cap already
     mine += want[q];
   }
   // exclusive scan of the per-thread sums

@@ -24,8 +24,8 @@
 //
 // Weight gradients need the points on the K axis, i.e. the transpose of the
 // register layout: each group's gradient and activation blocks are written to
-// a workgroup LDS tile (hi/lo bf16 pairs) and read back as 16x16x16 MFMA
-// operands.  Per-workgroup partial sums are reduced in a fixed order by
+// a workgroup LDS tile (bf16, rounded to nearest) and read back as 16x16x16
+// MFMA operands.  Per-workgroup partial sums are reduced in a fixed order by
 // mlp_reduce_rows, so results do not depend on scheduling.
 //
 // h0 = [x, enc] (input_concat) is held internally as [enc | x | 1 | 0 ...]
@@ -65,21 +65,23 @@ __device__ __forceinline__ void split_b(float v, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(v - (float)hi);
 }
 
-// Softplus(beta) with torch's threshold (value z above it) and its derivative.
+// Softplus(beta) with torch's threshold (value z above it) and its derivative
+// sigma(beta z) = t / (1 + t), t = e^(beta z).  Branch-free (both sides are
+// computed and one selected, so a wavefront never splits) on the hardware
+// exp2 / log2 / reciprocal: the backward's recompute, whose values enter bf16
+// products only.  A NaN z gives NaN value and derivative.
 struct Sp {
   float h, d;
 };
 __device__ __forceinline__ Sp softplus(float z, float beta, float inv_beta) {
+  constexpr float kLog2e = 1.4426950408889634f, kLn2 = 0.6931471805599453f;
   const float bz = z * beta;
+  const float t = __builtin_amdgcn_exp2f(bz * kLog2e);
+  const float u = 1.f + t;
+  const bool lin = bz > kSpThr;
   Sp s;
-  if (bz > kSpThr) {
-    s.h = z;
-    s.d = 1.f;
-  } else {
-    const float t = __expf(bz);
-    s.h = __logf(1.f + t) * inv_beta;
-    s.d = t / (1.f + t);
-  }
+  s.h = lin ? z : __builtin_amdgcn_logf(u) * (kLn2 * inv_beta);
+  s.d = lin ? 1.f : t * __builtin_amdgcn_rcpf(u);
   return s;
 }
 __device__ __forceinline__ float softplus_h(float z, float beta, float inv_beta) {
@@ -102,8 +104,8 @@ struct Cfg {
   static constexpr int nW1T = (NH == 2 ? MB * KBH : 0), nW0T = MD0 * KBH;
   static constexpr int kVecFloats = 3 * H + 4;  // b0, b1, wo, bo
   static constexpr size_t fwd_lds = (size_t)(nW0 + nW1) * 64 * 16 + kVecFloats * 4;
-  // two block tiles of 64 rows x (64 points + 4) dwords
-  static constexpr size_t tile_bytes = 2 * 64 * (16 * kWaves + 4) * 4;
+  // two block tiles of 64 rows x (64 points + 4) bf16
+  static constexpr size_t tile_bytes = 2 * 64 * (16 * kWaves + 4) * 2;
   static constexpr size_t bwd_lds =
       (size_t)(nW0 + nW1 + nW1T + nW0T) * 64 * 16 + kVecFloats * 4 + tile_bytes;
   __host__ __device__ static int n_params(int d) {
@@ -167,27 +169,37 @@ __device__ void build_frags(char* smem, const float* __restrict__ mlp, int E) {
   if (threadIdx.x == 0) vec[3 * H] = mlp[C::gBo(d)];
 }
 
-// The 8 h0 values of K block kb for point p (lane slots 8q..8q+7).
+// The 8 h0 values of K block kb for point p (lane slots 8q..8q+7).  With
+// vec4 (E % 4 == 0, 16-byte rows) the slots come in two 4-dim groups, each
+// either 4 encoding dims (one 16-byte load), the group [x, 1] that follows
+// them, or padding; otherwise one guarded load per slot.
 __device__ __forceinline__ void load_h0(const float* __restrict__ x, const float* __restrict__ enc,
                                         int E, bool vec4, int64_t p, bool ok, int kb, int q,
                                         float (&v)[8]) {
   const int c0 = 32 * kb + 8 * q;
-  if (ok && vec4 && c0 + 8 <= E) {
-    const float4* s = reinterpret_cast<const float4*>(enc + p * E + c0);
-    const float4 a = s[0], b = s[1];
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  if (vec4) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c4 = c0 + 4 * h;
+      float4 a = {0.f, 0.f, 0.f, 0.f};
+      if (ok && c4 < E) {
+        a = *reinterpret_cast<const float4*>(enc + p * E + c4);
+      } else if (ok && c4 == E) {
+        a = float4{x[p * 3], x[p * 3 + 1], x[p * 3 + 2], 1.f};
+      }
+      v[4 * h] = a.x;
+      v[4 * h + 1] = a.y;
+      v[4 * h + 2] = a.z;
+      v[4 * h + 3] = a.w;
+    }
     return;
   }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int c = c0 + j;
-    float t = 0.f;
-    if (ok) {
-      if (c < E) t = enc[p * E + c];
-      else if (c < E + 3) t = x[p * 3 + (c - E)];
-      else if (c == E + 3) t = 1.f;
-    }
+    const float* src = c < E ? enc + p * E + c : x + p * 3 + (c - E);
+    float t = (ok && c == E + 3) ? 1.f : 0.f;
+    if (ok && c < E + 3) t = *src;
     v[j] = t;
   }
 }
@@ -228,21 +240,22 @@ __device__ __forceinline__ float sum_r(float v) {  // over the 16 points of a q 
   return v;
 }
 
-// Forward of one 16-point group: z blocks of every layer (pre-activation) and
-// softplus values/derivatives.  Shared by the forward and backward kernels.
+// Forward of one 16-point group: softplus values/derivatives of the first
+// hidden layer kept (F), those of the last hidden layer handed unit by unit
+// to last(mb, e, value, derivative) as they are made, so they never occupy
+// registers together.  Shared by the forward and backward kernels.
 template <int H, int NH, int KB0>
 struct Fwd {
   static constexpr int MB = H / 16;
-  float h1[MB][4], d1[MB][4];  // softplus(z0), softplus'(z0)
-  float h2[MB][4], d2[MB][4];  // softplus(z1), softplus'(z1) (NH == 2)
+  float h1[MB][4], d1[MB][4];  // softplus(z0), softplus'(z0) (NH == 2)
 };
 
 // kLo: also multiply the lo parts of the activations (the forward whose values
 // are used); the backward's recompute needs only the hi parts.
-template <int H, int NH, int KB0, bool kDeriv, bool kLo = true>
+template <int H, int NH, int KB0, bool kDeriv, bool kLo, typename Last>
 __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1, const float* vec,
                                               const float (&h0)[KB0][8], float beta, float inv_beta,
-                                              Fwd<H, NH, KB0>& F) {
+                                              Fwd<H, NH, KB0>& F, Last&& last) {
   using C = Cfg<H, NH, KB0>;
   const int l = threadIdx.x & 63, q = l >> 4;
   half8 bh[KB0], bl[KB0];
@@ -267,12 +280,14 @@ __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float zz = z[e];  // b0 enters through the constant h0 dim
-      if (kDeriv) {
-        const Sp s = softplus(zz, beta, inv_beta);
-        F.h1[mb][e] = s.h;
-        F.d1[mb][e] = s.d;
+      Sp sp;
+      if (kDeriv) sp = softplus(zz, beta, inv_beta);
+      else sp = Sp{softplus_h(zz, beta, inv_beta), 0.f};
+      if (NH == 2) {
+        F.h1[mb][e] = sp.h;
+        F.d1[mb][e] = sp.d;
       } else {
-        F.h1[mb][e] = softplus_h(zz, beta, inv_beta);
+        last(mb, e, sp.h, sp.d);
       }
     }
   }
@@ -293,11 +308,10 @@ __device__ __forceinline__ void forward_group(const half8* fW0, const half8* fW1
       for (int e = 0; e < 4; ++e) {
         const float zz = z[e] + vec[H + 16 * mb + 4 * q + e];
         if (kDeriv) {
-          const Sp s = softplus(zz, beta, inv_beta);
-          F.h2[mb][e] = s.h;
-          F.d2[mb][e] = s.d;
+          const Sp sp = softplus(zz, beta, inv_beta);
+          last(mb, e, sp.h, sp.d);
         } else {
-          F.h2[mb][e] = softplus_h(zz, beta, inv_beta);
+          last(mb, e, softplus_h(zz, beta, inv_beta), 0.f);
         }
       }
     }
@@ -335,64 +349,45 @@ mlp16_fwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
 #pragma unroll
     for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, E, vec4, p, ok, kb, q, h0[kb]);
     Fwd<H, NH, KB0> F;
-    forward_group<H, NH, KB0, false>(fW0, fW1, vec, h0, beta, inv_beta, F);
     float s = 0.f;
-#pragma unroll
-    for (int mb = 0; mb < C::MB; ++mb)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        s += vec[2 * H + 16 * mb + 4 * q + e] * (NH == 2 ? F.h2[mb][e] : F.h1[mb][e]);
+    forward_group<H, NH, KB0, false, true>(
+        fW0, fW1, vec, h0, beta, inv_beta, F,
+        [&](int mb, int e, float h, float) { s += vec[2 * H + 16 * mb + 4 * q + e] * h; });
     s = sum_q(s) + vec[3 * H];
     if (q == 0 && ok) sdf_out[p] = s;
   }
 }
 
-// Block tiles: rows of (hi, lo) bf16 pairs over the 64 points of a block step
-// (wavefront w owns columns 16w..16w+15).  Row pitch 68 dwords keeps the
-// writes of a half-wavefront on distinct banks.
+// Block tiles: rows of bf16 over the 64 points of a block step (wavefront w
+// owns columns 16w..16w+15).  Row pitch 68 bf16 (34 dwords): the writes of a
+// wavefront (4 rows 4 apart, 16 columns each) fall on 32 distinct banks, and
+// a 16-lane quarter's 8-byte operand reads (16 rows) on 32 distinct banks.
 constexpr int kBlkPts = 16 * kWaves;
 constexpr int kTileP = kBlkPts + 4;
 
-__device__ __forceinline__ uint32_t pack_pair(float v) {
-  __bf16 hi, lo;
-  split_b(v, hi, lo);
-  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
-         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
-}
 // Writes D-layout blocks (this wavefront's point r, rows 16 mb + 4q + e).
 template <int MB>
-__device__ __forceinline__ void tile_store(uint32_t* t, const float (&v)[MB][4]) {
+__device__ __forceinline__ void tile_store(__bf16* t, const float (&v)[MB][4]) {
   const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
   const int col = 16 * (threadIdx.x >> 6) + r;
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) t[(16 * mb + 4 * q + e) * kTileP + col] = pack_pair(v[mb][e]);
+    for (int e = 0; e < 4; ++e) t[(16 * mb + 4 * q + e) * kTileP + col] = (__bf16)v[mb][e];
 }
-// 16x16x16 operand: tile row `row`, points 16k + 4q .. +3, as hi and lo.
-__device__ __forceinline__ void tile_operand(const uint32_t* t, int row, int k, bf16x4& hi,
-                                             bf16x4& lo) {
+// 16x16x16 operand: tile row `row`, points 16k + 4q .. +3 (one 8-byte read).
+__device__ __forceinline__ bf16x4 tile_operand(const __bf16* t, int row, int k) {
   const int q = (threadIdx.x & 63) >> 4;
-  const uint4 w = *reinterpret_cast<const uint4*>(t + row * kTileP + 16 * k + 4 * q);
-  const uint32_t d[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    hi[j] = __builtin_bit_cast(__bf16, (uint16_t)(d[j] & 0xffffu));
-    lo[j] = __builtin_bit_cast(__bf16, (uint16_t)(d[j] >> 16));
-  }
+  return *reinterpret_cast<const bf16x4*>(t + row * kTileP + 16 * k + 4 * q);
 }
 // acc += A[16a .. 16a+15][pts] * B[16b .. 16b+15][pts]^T over the step's points
-// (bf16 products of the hi parts: a gradient sum, ~2^-9 relative per term).
-__device__ __forceinline__ floatx4 tile_mma(const uint32_t* tA, int a, const uint32_t* tB, int b,
+// (bf16 products: a gradient sum, ~2^-9 relative per term).
+__device__ __forceinline__ floatx4 tile_mma(const __bf16* tA, int a, const __bf16* tB, int b,
                                             floatx4 acc) {
   const int r = threadIdx.x & 15;
 #pragma unroll
-  for (int k = 0; k < kWaves; ++k) {
-    bf16x4 ah, al, bh, bl;
-    tile_operand(tA, 16 * a + r, k, ah, al);
-    tile_operand(tB, 16 * b + r, k, bh, bl);
-    acc = mma_b4(ah, bh, acc);
-  }
+  for (int k = 0; k < kWaves; ++k)
+    acc = mma_b4(tile_operand(tA, 16 * a + r, k), tile_operand(tB, 16 * b + r, k), acc);
   return acc;
 }
 
@@ -425,9 +420,10 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
   const bf16x8* fW1T = reinterpret_cast<const bf16x8*>(fW1 + C::nW1 * 64);
   const bf16x8* fW0T = fW1T + C::nW1T * 64;
   const float* vec = reinterpret_cast<const float*>(fW0T + C::nW0T * 64);
-  uint32_t* tileA = reinterpret_cast<uint32_t*>(smem + C::bwd_lds - C::tile_bytes);
-  uint32_t* tileB = tileA + 64 * kTileP;
-  const int wid = threadIdx.x >> 6;
+  __bf16* tileA = reinterpret_cast<__bf16*>(smem + C::bwd_lds - C::tile_bytes);
+  __bf16* tileB = tileA + 64 * kTileP;
+  // wavefront index as a scalar: the block-ownership tests below are uniform
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const float inv_beta = 1.f / beta;
   const int64_t n = dev_count(n_cap, n_dev);
   const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
@@ -449,24 +445,24 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
   for (int64_t step = blockIdx.x; step < n_steps; step += gridDim.x) {
     const int64_t p = (step * kWaves + wid) * 16 + r;
     const bool ok = p < n;
+    // the lane's quarter index, opaque to the compiler in every step: the
+    // per-slot boundary tests of the h0 loads and gradient stores derive
+    // from it and are then evaluated where used, instead of being hoisted
+    // out of the loop as dozens of long-lived execution masks (which spill)
+    int qs = q, Es = E, v4s = vec4;
+    asm volatile("" : "+v"(qs), "+s"(Es), "+s"(v4s));
     float h0[KB0][8];
 #pragma unroll
-    for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, E, vec4, p, ok, kb, q, h0[kb]);
+    for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, Es, v4s, p, ok, kb, qs, h0[kb]);
     const float gs = ok ? g_sdf[p] : 0.f;
     Fwd<H, NH, KB0> F;
-    forward_group<H, NH, KB0, true, false>(fW0, fW1, vec, h0, beta, inv_beta, F);
     if (q == 0) dbo += gs;
     float gz[MB][4];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int u = 16 * mb + 4 * q + e;
-        const float hL = NH == 2 ? F.h2[mb][e] : F.h1[mb][e];
-        const float dL = NH == 2 ? F.d2[mb][e] : F.d1[mb][e];
-        dwo[mb][e] += gs * hL;
-        gz[mb][e] = gs * vec[2 * H + u] * dL;
-      }
+    forward_group<H, NH, KB0, true, false>(
+        fW0, fW1, vec, h0, beta, inv_beta, F, [&](int mb, int e, float hL, float dL) {
+          dwo[mb][e] += gs * hL;
+          gz[mb][e] = gs * vec[2 * H + 16 * mb + 4 * q + e] * dL;
+        });
     if (NH == 2) {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb)
@@ -502,7 +498,7 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
     for (int kb = 0; kb < KB0; ++kb)
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        tileB[(32 * kb + 8 * q + j) * kTileP + 16 * wid + r] = pack_pair(h0[kb][j]);
+        tileB[(32 * kb + 8 * q + j) * kTileP + 16 * wid + r] = (__bf16)h0[kb][j];
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < T0; ++t) {
@@ -522,21 +518,24 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
           const bf16x8 w = fW0T[(md * KBH + kb) * 64 + l];
           acc = mma_b(w, gh[kb], acc);
         }
-        if (ok) {
-          const int c0 = 16 * md + 4 * q;  // internal dims c0..c0+3 of point p
-          if (vec4 && c0 + 4 <= E) {
+        const int c0 = 16 * md + 4 * qs;  // internal dims c0..c0+3 of point p
+        if (v4s) {  // 4-dim groups: encoding dims, or [x, 1]
+          if (ok && c0 < Es) {
             if (g_enc)
-              *reinterpret_cast<float4*>(g_enc + p * E + c0) = float4{acc[0], acc[1], acc[2], acc[3]};
-          } else {
+              *reinterpret_cast<float4*>(g_enc + p * Es + c0) =
+                  float4{acc[0], acc[1], acc[2], acc[3]};
+          } else if (ok && c0 == Es && g_x) {
+            g_x[p * 3] = acc[0];
+            g_x[p * 3 + 1] = acc[1];
+            g_x[p * 3 + 2] = acc[2];
+          }
+        } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int c = c0 + e;
-              if (c < E) {
-                if (g_enc) g_enc[p * E + c] = acc[e];
-              } else if (c < E + 3) {
-                if (g_x) g_x[p * 3 + (c - E)] = acc[e];
-              }
-            }
+          for (int e = 0; e < 4; ++e) {
+            const int c = c0 + e;
+            float* dst = c < Es ? g_enc + p * Es + c : g_x + p * 3 + (c - Es);
+            const bool want = c < Es ? g_enc != nullptr : (c < Es + 3 && g_x != nullptr);
+            if (ok && want) *dst = acc[e];
           }
         }
       }

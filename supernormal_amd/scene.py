@@ -193,11 +193,21 @@ def render_normal_map(scene, K, pose, W, H, device, n_iter=160, chunk=1 << 20):
         t = torch.where(ok, (-b - disc.clamp_min(0).sqrt()) / 2, torch.zeros_like(b)).clamp_min(0)
         tfar = torch.where(ok, (-b + disc.clamp_min(0).sqrt()) / 2, torch.zeros_like(b))
         hit = torch.zeros_like(ok)
-        for _ in range(n_iter):
-            p = o + d * t[:, None]
-            f = scene(p)
-            hit = hit | (ok & (f.abs() < 1e-5) & (t <= tfar))
-            t = torch.where(hit | ~ok, t, t + f * 0.9)
+        # sphere tracing over the rays still marching: every 8 steps the rays
+        # that hit, missed the unit sphere or left it (t > tfar: the scene is
+        # positive outside it, so t only grows) are dropped from the set; each
+        # ray's own sequence of steps is unchanged
+        act = torch.nonzero(ok).squeeze(1)
+        for it in range(n_iter):
+            if it % 8 == 0 and it:
+                act = act[~hit[act] & (t[act] <= tfar[act])]
+                if act.numel() == 0:
+                    break
+            ta = t[act]
+            f = scene(o + d[act] * ta[:, None])
+            h = hit[act] | ((f.abs() < 1e-5) & (ta <= tfar[act]))
+            hit[act] = h
+            t[act] = torch.where(h, ta, ta + f * 0.9)
         p = o + d * t[:, None]
         hit = hit & (t <= tfar)
         n = scene_normal(scene, p)

@@ -4,7 +4,8 @@
 (models/fields.py:76-119) with hand-written backward passes:
   * sdf:               x -> hash encode -> MLP -> sdf;  backward: MLP backward
                        (weight gradients summed in fixed order) -> dL/denc ->
-                       hash backward (fp32 atomics into the table gradient).
+                       hash backward (bucket records summed in fixed point, with
+                       a workspace kept between calls).
   * sdf_and_gradient:  also the analytic x-gradient
                        grad = dsdf/dx|_direct + sum_l dsdf/denc_l * denc_l/dx;
                        its backward is the double backward the reference gets
@@ -38,6 +39,12 @@ class SDFConfig:
         # 'fp16': MLP on the fp16/bf16 matrix cores (sn_mlp16_*) for the sdf
         # value and its backward; the analytic gradient stays on fp32 kernels
         self.precision = precision
+        # the hash-grid backward's bucket workspace, kept between calls
+        from .tcnn import BucketWorkspace
+        self.workspace = BucketWorkspace()
+
+    def ws(self, n_active, n, device):
+        return self.workspace.get(self.hash_args(n_active), n, device)
 
     def hash_args(self, n_active):
         g = self.grid
@@ -106,7 +113,8 @@ class _SDFFn(torch.autograd.Function):
                  cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
                  ws.numel(), s)
         g_params = torch.zeros_like(params)
-        hashgrid_bwd(x, n, None, params, cfg.hash_args(ctx.n_active), g_enc, g_params, None, s)
+        hashgrid_bwd(x, n, None, params, cfg.hash_args(ctx.n_active), g_enc, g_params, None, s,
+                     ws=cfg.ws(ctx.n_active, n, dev))
         return None, g_params, g_mlp, None, None
 
 
@@ -149,7 +157,7 @@ class _SDFGradFn(torch.autograd.Function):
                  cfg.n_layers, cfg.beta, ptr(g_sdf), ptr(g_enc), None, ptr(g_mlp), 0, ptr(ws),
                  ws.numel(), s)
             hashgrid_bwd(x, n, None, params, cfg.hash_args(ctx.n_active), g_enc, g_params,
-                         None, s)
+                         None, s, ws=cfg.ws(ctx.n_active, n, dev))
             return None, g_params, g_mlp, None, None
         v = g_grad.contiguous()
         hargs = cfg.hash_args(ctx.n_active)
@@ -166,7 +174,8 @@ class _SDFGradFn(torch.autograd.Function):
              ptr(g_mlp), 0, ptr(ws), ws.numel(), s)
         ge = g_h0[:, 3:].contiguous()
         ged = g_h0d[:, 3:].contiguous()
-        hashgrid_bwd(x, n, None, params, hargs, ge, g_params, None, s)
+        hashgrid_bwd(x, n, None, params, hargs, ge, g_params, None, s,
+                     ws=cfg.ws(ctx.n_active, n, dev))
         call("sn_hashgrid_bwd_bwd", ptr(x), n, ptr(params), *hargs, ptr(ged), ptr(v), None,
              ptr(g_params), None, s)
         return None, g_params, g_mlp, None, None

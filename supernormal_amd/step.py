@@ -32,8 +32,10 @@ runs ahead across it); the active hash levels are a device scalar, so a
 bandwidth increase changes no launch.  While a march is pending, the buffers it writes
 (inputs, counts, packed0, n0, ri0/t00/t10) already hold the next iteration's.
 
-Only the 'dfd' gradient method is supported here (the analytic gradient uses
-FusedStep), and max_samples_per_ray must be set (it bounds the capacity).
+Both gradient methods run here: 'dfd' (finite differences of the SDF at the
+patch points) and 'ad' (the analytic x-gradient and its double backward, on
+the fp32 MLP kernels).  max_samples_per_ray must be set (it bounds the
+capacity).
 """
 from __future__ import annotations
 

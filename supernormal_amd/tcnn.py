@@ -90,17 +90,46 @@ def hashgrid_bwd(x, n, n_dev, params, gargs, genc, gparams, gx, stream, ws=None,
          0 if ws is None else ws.numel(), stream)
 
 
+class BucketWorkspace:
+    """A bucket workspace kept between backward calls (one per encoder): it
+    carries the previous call's list sizes, from which each call shares out
+    its record pool, so the layout follows the data and lists seldom
+    overflow (records beyond a list take order-dependent fp32 atomics).
+    Grown, with a quarter in hand, when a call has more points than it was
+    sized for; a call with fewer points uses the whole buffer (the pool size,
+    and with it the layout, stays the same).  Calls sharing it must not
+    overlap (one stream)."""
+
+    def __init__(self):
+        self.buf = None
+        self.key = None
+
+    def get(self, gargs, n, device):
+        nb = bucket_workspace_bytes(gargs, n)
+        if not nb:
+            return None
+        key = (tuple(gargs[:5]), torch.device(device))
+        if self.buf is None or self.key != key or self.buf.numel() < nb:
+            self.buf = None
+            nb = bucket_workspace_bytes(gargs, n + n // 4)
+            self.buf = torch.empty(nb, dtype=torch.uint8, device=device)
+            self.key = key
+        return self.buf
+
+
 class _HashBackwardFn(torch.autograd.Function):
     """(dL/denc, x, params) -> (dL/dx, dL/dparams); differentiable in its first
     output (tcnn's bwd_bwd_input)."""
 
     @staticmethod
-    def forward(ctx, genc, x, params, g, n_active, want_x):
+    def forward(ctx, genc, x, params, g, n_active, want_x, wsh=None):
         genc = genc.contiguous()
         gparams = torch.zeros_like(params)
         gx = torch.empty_like(x) if want_x else None
-        hashgrid_bwd(x, x.shape[0], None, params, _gargs(g, n_active), genc, gparams, gx,
-                     stream_of(x.device))
+        gargs = _gargs(g, n_active)
+        ws = wsh.get(gargs, x.shape[0], x.device) if wsh is not None else None
+        hashgrid_bwd(x, x.shape[0], None, params, gargs, genc, gparams, gx,
+                     stream_of(x.device), ws=ws)
         ctx.save_for_backward(genc, x, params)
         ctx.g, ctx.n_active = g, n_active
         if gx is None:
@@ -122,27 +151,27 @@ class _HashBackwardFn(torch.autograd.Function):
             call("sn_hashgrid_bwd_bwd", ptr(x), x.shape[0], ptr(params),
                  *_gargs(ctx.g, ctx.n_active), ptr(genc), ptr(ggx), ptr(gge), ptr(gp), ptr(gx2),
                  stream_of(x.device))
-        return gge, gx2, gp, None, None, None
+        return gge, gx2, gp, None, None, None, None
 
 
 class _HashEncodeFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, params, g, n_active):
+    def forward(ctx, x, params, g, n_active, wsh=None):
         x = x.contiguous()
         enc = torch.empty((x.shape[0], g["n_levels"] * g["n_features"]), dtype=torch.float32,
                           device=x.device)
         call("sn_hashgrid_fwd", ptr(x), x.shape[0], ptr(params), *_gargs(g, n_active), ptr(enc),
              stream_of(x.device))
         ctx.save_for_backward(x, params)
-        ctx.g, ctx.n_active = g, n_active
+        ctx.g, ctx.n_active, ctx.wsh = g, n_active, wsh
         return enc
 
     @staticmethod
     def backward(ctx, genc):
         x, params = ctx.saved_tensors
         gx, gp = _HashBackwardFn.apply(genc, x, params, ctx.g, ctx.n_active,
-                                       bool(ctx.needs_input_grad[0]))
-        return (gx if ctx.needs_input_grad[0] else None), gp, None, None
+                                       bool(ctx.needs_input_grad[0]), ctx.wsh)
+        return (gx if ctx.needs_input_grad[0] else None), gp, None, None, None
 
 
 class Encoding(nn.Module):
@@ -167,6 +196,8 @@ class Encoding(nn.Module):
         init = (torch.rand(self.n_entries * g["n_features"], generator=gen) * 2 - 1) * 1e-4
         self.params = nn.Parameter(init.to(torch.float32), requires_grad=True)
         self.n_active_levels = g["n_levels"]
+        # the backward's bucket workspace, kept between calls
+        self.workspace = BucketWorkspace()
 
     @property
     def n_params(self):
@@ -187,7 +218,7 @@ class Encoding(nn.Module):
             f"expected input of shape [N, {self.n_input_dims}], got {tuple(x.shape)}"
         x = x.to(torch.float32)
         n_active = max(0, min(int(self.n_active_levels), self.grid["n_levels"]))
-        out = _HashEncodeFn.apply(x, self.params, self.grid, n_active)
+        out = _HashEncodeFn.apply(x, self.params, self.grid, n_active, self.workspace)
         return out if self.dtype == torch.float32 else out.to(self.dtype)
 
     def extra_repr(self):

@@ -189,6 +189,40 @@ def test_bucket_pool_follows_previous_counts(dev):
     assert torch.equal(outs[1], outs[2])
 
 
+def test_encoding_keeps_its_workspace(dev):
+    """tcnn.Encoding keeps one bucket workspace between backward calls: the
+    same buffer serves every call (fewer points included), the level-0 list
+    that overflows the fresh even split no longer overflows from the second
+    call on, and from then on the table gradient repeats bit for bit and
+    agrees with the atomic path."""
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=11)
+    g = torch.Generator().manual_seed(12)
+    n = 12000
+    x = (torch.rand(n, 3, generator=g) * 2 - 1).to(dev)
+    genc = torch.randn(n, 32, generator=g).to(dev)
+    g_ref = torch.zeros_like(e.params)
+    call("sn_hashgrid_bwd_dev", ptr(x), n, None, ptr(e.params), 16, 2, 19, 16, 1.38, 16, None,
+         ptr(genc), ptr(g_ref), None, stream_of(dev))
+    scale = float(g_ref.abs().max())
+    outs, bufs = [], []
+    for k in range(4):
+        m = n if k < 3 else n // 2
+        e.params.grad = None
+        (e(x[:m]) * genc[:m]).sum().backward()
+        bufs.append(e.workspace.buf.data_ptr())
+        outs.append(e.params.grad.clone())
+    assert len(set(bufs)) == 1
+    for gp in outs[:3]:
+        torch.testing.assert_close(gp, g_ref, rtol=1e-4, atol=1e-5 * scale)
+    assert torch.equal(outs[1], outs[2])
+    half = torch.zeros_like(e.params)
+    call("sn_hashgrid_bwd_dev", ptr(x), n // 2, None, ptr(e.params), 16, 2, 19, 16, 1.38, 16,
+         None, ptr(genc), ptr(half), None, stream_of(dev))
+    torch.testing.assert_close(outs[3], half, rtol=1e-4, atol=1e-5 * scale)
+
+
 def test_double_backward_vs_fp64(dev):
     """L = sum(v * dL1/dx) with dL1/dx = autograd.grad(sum(g * enc), x, create_graph=True):
     d L / d params, d L / d g (through enc's upstream gradient) and d L / d x."""
