@@ -483,6 +483,8 @@ def plain_full(dev, workload, seed, max_samples, log=None):
 
 
 def main(argv=None):
+    import faulthandler
+    faulthandler.enable()  # a crash prints the Python stack of every thread
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed iterations (end of schedule)")

@@ -47,6 +47,13 @@ int64_t make_hashgrid_desc(int32_t n_levels, int32_t log2_T, int32_t base_res,
   return (int64_t)off;
 }
 
+// Levels per step of the paired forward (F = 2, 16-byte output rows): 2 or 4.
+#ifndef SN_FWD_LEVELS
+#define SN_FWD_LEVELS 4
+#endif
+constexpr int kFwdLevels = SN_FWD_LEVELS;
+static_assert(kFwdLevels == 2 || kFwdLevels == 4, "forward level group of 2 or 4");
+
 // Table element type TP: float (the parameters) or _Float16 (a copy refreshed
 // by the optimizer step: half the gather bytes; blending stays fp32).
 template <int F, typename TP, bool kPair>
@@ -61,6 +68,51 @@ __device__ __forceinline__ void hash_fwd_point(int64_t i, const float* __restric
   // the L2 does not keep between levels
   const bool vec_out = F == 2 && kPair && (d.n_levels & 3) == 0 &&
                        (reinterpret_cast<uintptr_t>(enc) & 15) == 0;
+  if constexpr (F == 2 && kPair) {
+    if (d.n_levels % kFwdLevels == 0 && (reinterpret_cast<uintptr_t>(enc) & 15) == 0) {
+      // kFwdLevels levels at a time: their corner gathers are all issued
+      // before any is used (that many times the loads in flight of a
+      // level-by-level loop; the gathers are what this kernel waits on), then
+      // blended in the same order as below
+      typedef TP pair_t __attribute__((ext_vector_type(2)));
+      constexpr int G = kFwdLevels;
+      for (int l0 = 0; l0 < d.n_levels; l0 += G) {
+        uint32_t idx[G][8];
+        float w[G][8];
+        pair_t v[G][8];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          if (l0 + j < na) {  // uniform
+            float fr[3];
+            hg_level(p, d, l0 + j, idx[j], w[j], fr);
+            const pair_t* tab2 =
+                reinterpret_cast<const pair_t*>(params + (int64_t)d.offset[l0 + j] * F);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[j][c] = tab2[idx[j][c]];
+          }
+        }
+        float o[2 * G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          float a0 = 0.f, a1 = 0.f;
+          if (l0 + j < na) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+              a0 = fmaf(w[j][c], (float)v[j][c].x, a0);
+              a1 = fmaf(w[j][c], (float)v[j][c].y, a1);
+            }
+          }
+          o[2 * j] = a0;
+          o[2 * j + 1] = a1;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j += 2)
+          *reinterpret_cast<float4*>(out + (l0 + j) * F) =
+              make_float4(o[2 * j], o[2 * j + 1], o[2 * j + 2], o[2 * j + 3]);
+      }
+      return;
+    }
+  }
   float o0 = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f, o4 = 0.f, o5 = 0.f, o6 = 0.f, o7 = 0.f;
   for (int l = 0; l < d.n_levels; ++l) {
     float acc[F];
@@ -571,15 +623,42 @@ __device__ __forceinline__ float dpp_f(float v) {
                                                     true));
 }
 
-// Sum over the wavefront's 64 lanes, returned uniform (via lane 63).
-__device__ __forceinline__ float wave_sum(float v) {
-  v += dpp_f<0x111>(v);
-  v += dpp_f<0x112>(v);
-  v += dpp_f<0x114>(v);
-  v += dpp_f<0x118>(v);
-  v += dpp_f<0x142, 0xa>(v);
-  v += dpp_f<0x143, 0xc>(v);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+// One butterfly step of a reduce-scatter inside each 16-lane row: a[0..2H)
+// -> a[0..H), the lane keeping the upper half when `upper`, plus the copy of
+// that half held by its partner (the lane whose `upper` differs, reached by
+// the DPP pattern kCtrl: row_mirror, row_half_mirror or a quad permutation).
+template <int H, int kCtrl>
+__device__ __forceinline__ void rs_step(float* a, bool upper) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float keep = upper ? a[k + H] : a[k];
+    const float send = upper ? a[k] : a[k + H];
+    a[k] = keep + dpp_f<kCtrl>(send);
+  }
+}
+
+// Wavefront sums of the N = 8F values a[] (a group's corner contributions,
+// zero outside the group), all returned to lane ld's r[c][f] (a[2c + f] for
+// F = 2, a[c] for F = 1): a reduce-scatter inside each row leaves lane i
+// with value i % N summed over its row (or half-row), the rows' partial
+// sums are added across lanes (row rotation by 8 for N = 8, a swizzle
+// across 16, a permute across 32), and lane ld reads the N totals.  About
+// 3N + 2 N lane operations instead of N full-wavefront reductions.
+template <int F>
+__device__ __forceinline__ void group_sums(float (&a)[8 * F], int lane, int ld, float (&r)[8][F]) {
+  if constexpr (F == 2) rs_step<8, 0x140>(a, lane & 8);  // row_mirror: lane i <-> 15 - i
+  rs_step<4, 0x141>(a, lane & 4);                        // row_half_mirror: i <-> 7 - i
+  rs_step<2, 0x4E>(a, lane & 2);                         // quad_perm [2,3,0,1]
+  rs_step<1, 0xB1>(a, lane & 1);                         // quad_perm [1,0,3,2]
+  float v = a[0];
+  if constexpr (F == 1) v += dpp_f<0x128>(v);  // row_ror:8
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));  // lane ^ 16
+  v += __shfl_xor(v, 32, 64);
+#pragma unroll
+  for (int j = 0; j < 8 * F; ++j) {
+    const float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    if (lane == ld) r[j / F][j % F] = t;
+  }
 }
 
 // Records of one level wait in LDS, ordered by bucket, before they are
@@ -671,13 +750,12 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         const bool in = ((pend >> lane) & 1ull) && cell[0] == c0 && cell[1] == c1 && cell[2] == c2;
         const uint64_t m = __ballot(in);
         if (__popcll(m) >= kMinGroup) {
+          float a[8 * F];
 #pragma unroll
           for (int c = 0; c < 8; ++c)
 #pragma unroll
-            for (int f = 0; f < F; ++f) {
-              const float sum = wave_sum(in ? w[c] * gl[f] : 0.f);
-              if (lane == ld) rv[c][f] = sum;
-            }
+            for (int f = 0; f < F; ++f) a[c * F + f] = in ? w[c] * gl[f] : 0.f;
+          group_sums<F>(a, lane, ld, rv);
           if (in && lane != ld) has = false;
           singles = 0;
         } else if (++singles >= kMaxSingles) {

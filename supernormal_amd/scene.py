@@ -170,17 +170,17 @@ def scene_normal(scene, p, eps=1e-4):
     return g / g.norm(dim=-1, keepdim=True).clamp_min(1e-12)
 
 
-@torch.no_grad()
-def render_normal_map(scene, K, pose, W, H, device, n_iter=160, chunk=1 << 20):
-    """World-space unit normals [H,W,3] and mask [H,W] of the first hit."""
+def _pixel_rays(K, pose, W, H, device, chunk):
+    """Per chunk of pixels (row-major): origin [3], unit directions, and the
+    entry / exit distances of the unit sphere with a flag of the rays that
+    meet it."""
     Kinv = torch.inverse(torch.as_tensor(K[:3, :3], dtype=torch.float64, device=device))
     R = torch.as_tensor(pose[:3, :3], dtype=torch.float64, device=device)
     o = torch.as_tensor(pose[:3, 3], dtype=torch.float64, device=device)
     ys, xs = torch.meshgrid(torch.arange(H, device=device, dtype=torch.float64),
                             torch.arange(W, device=device, dtype=torch.float64), indexing="ij")
     pix = torch.stack([xs, ys, torch.ones_like(xs)], -1).reshape(-1, 3)
-    normals = torch.zeros(H * W, 3, dtype=torch.float64, device=device)
-    mask = torch.zeros(H * W, dtype=torch.bool, device=device)
+    out = []
     for s in range(0, H * W, chunk):
         d = pix[s:s + chunk] @ Kinv.T
         d = d / d.norm(dim=-1, keepdim=True)
@@ -192,28 +192,84 @@ def render_normal_map(scene, K, pose, W, H, device, n_iter=160, chunk=1 << 20):
         ok = disc > 0
         t = torch.where(ok, (-b - disc.clamp_min(0).sqrt()) / 2, torch.zeros_like(b)).clamp_min(0)
         tfar = torch.where(ok, (-b + disc.clamp_min(0).sqrt()) / 2, torch.zeros_like(b))
-        hit = torch.zeros_like(ok)
-        # sphere tracing over the rays still marching: every 8 steps the rays
-        # that hit, missed the unit sphere or left it (t > tfar: the scene is
-        # positive outside it, so t only grows) are dropped from the set; each
-        # ray's own sequence of steps is unchanged
-        act = torch.nonzero(ok).squeeze(1)
-        for it in range(n_iter):
-            if it % 8 == 0 and it:
-                act = act[~hit[act] & (t[act] <= tfar[act])]
-                if act.numel() == 0:
-                    break
-            ta = t[act]
-            f = scene(o + d[act] * ta[:, None])
-            h = hit[act] | ((f.abs() < 1e-5) & (ta <= tfar[act]))
-            hit[act] = h
-            t[act] = torch.where(h, ta, ta + f * 0.9)
-        p = o + d * t[:, None]
-        hit = hit & (t <= tfar)
-        n = scene_normal(scene, p)
-        normals[s:s + chunk] = torch.where(hit[:, None], n, torch.zeros_like(n))
-        mask[s:s + chunk] = hit
-    return normals.reshape(H, W, 3), mask.reshape(H, W)
+        out.append((o, d, t, tfar, ok))
+    return out
+
+
+def _sphere_trace(scene, o, d, t, tfar, ok, n_iter):
+    """First hits along the rays (o [N,3] or [3], d [N,3]): t updated in place;
+    returns the hit flags.  Every 8 steps the rays that hit, missed the unit
+    sphere or left it (t > tfar: the scene is positive outside it, so t only
+    grows) are dropped from the set; each ray's own sequence of steps does
+    not depend on which other rays are traced with it."""
+    hit = torch.zeros_like(ok)
+    act = torch.nonzero(ok).squeeze(1)
+    per_ray_o = o.dim() == 2
+    for it in range(n_iter):
+        if it % 8 == 0 and it:
+            act = act[~hit[act] & (t[act] <= tfar[act])]
+            if act.numel() == 0:
+                break
+        ta = t[act]
+        f = scene((o[act] if per_ray_o else o) + d[act] * ta[:, None])
+        h = hit[act] | ((f.abs() < 1e-5) & (ta <= tfar[act]))
+        hit[act] = h
+        t[act] = torch.where(h, ta, ta + f * 0.9)
+    return hit
+
+
+@torch.no_grad()
+def render_normal_maps(scene, K, poses, W, H, device, n_iter=160, chunk=1 << 20,
+                       trace_rays=1 << 23):
+    """World-space unit normals [H,W,3] and masks [H,W] of the first hits of
+    several views: the rays of the views are sphere-traced together, up to
+    trace_rays at a time (a step of the trace is ~a few hundred small
+    launches whatever the ray count, so fewer, larger batches take less
+    time); per ray the result is the one render_normal_map gives."""
+    views = [_pixel_rays(K, pose, W, H, device, chunk) for pose in poses]
+    flat = [(v, k) for v, pieces in enumerate(views) for k in range(len(pieces))]
+    hits = {}
+    i = 0
+    while i < len(flat):
+        group, n = [], 0
+        while i < len(flat) and (not group or n + len(views[flat[i][0]][flat[i][1]][1]) <= trace_rays):
+            group.append(flat[i])
+            n += len(views[flat[i][0]][flat[i][1]][1])
+            i += 1
+        pieces = [views[v][k] for v, k in group]
+        O = torch.cat([pc[0].expand(len(pc[1]), 3) for pc in pieces])
+        D = torch.cat([pc[1] for pc in pieces])
+        T = torch.cat([pc[2] for pc in pieces])
+        TF = torch.cat([pc[3] for pc in pieces])
+        OK = torch.cat([pc[4] for pc in pieces])
+        hit = _sphere_trace(scene, O, D, T, TF, OK, n_iter)
+        s0 = 0
+        for (v, k), pc in zip(group, pieces):
+            m = len(pc[1])
+            hits[(v, k)] = (T[s0:s0 + m], hit[s0:s0 + m])
+            s0 += m
+    out = []
+    for v, pieces in enumerate(views):
+        normals = torch.zeros(H * W, 3, dtype=torch.float64, device=device)
+        mask = torch.zeros(H * W, dtype=torch.bool, device=device)
+        s = 0
+        for k, (o, d, _, tfar, _) in enumerate(pieces):
+            t, hit = hits[(v, k)]
+            p = o + d * t[:, None]
+            hit = hit & (t <= tfar)
+            nrm = scene_normal(scene, p)
+            m = len(d)
+            normals[s:s + m] = torch.where(hit[:, None], nrm, torch.zeros_like(nrm))
+            mask[s:s + m] = hit
+            s += m
+        out.append((normals.reshape(H, W, 3), mask.reshape(H, W)))
+    return out
+
+
+@torch.no_grad()
+def render_normal_map(scene, K, pose, W, H, device, n_iter=160, chunk=1 << 20):
+    """World-space unit normals [H,W,3] and mask [H,W] of the first hit."""
+    return render_normal_maps(scene, K, [pose], W, H, device, n_iter, chunk)[0]
 
 
 # --------------------------------------------------------------------------- #
@@ -234,8 +290,9 @@ class Dataset:
         poses = ring_cameras(n_views, radius, elevations, seed)
         g = torch.Generator(device="cpu").manual_seed(seed)
         normals, masks = [], []
+        maps = render_normal_maps(scene, K, poses, W, H, self.device)
         for i in range(n_views):
-            n, m = render_normal_map(scene, K, poses[i], W, H, self.device)
+            n, m = maps[i]
             if noise_std > 0:
                 noise = torch.randn(H, W, 3, generator=g, dtype=torch.float64).to(self.device)
                 n = n + noise_std * noise
