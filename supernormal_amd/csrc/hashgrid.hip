@@ -453,8 +453,8 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    values).  Lanes of a wavefront whose points lie in the same cell (common
 //    on coarse and middle levels: a wavefront holds neighbouring samples of
 //    a few rays of one patch) are summed first with cross-lane (DPP)
-//    reductions and leave one record per corner; the other lanes leave one
-//    record per corner each.
+//    reduce-scatters and leave one record per corner; the other lanes leave
+//    one record per corner each.
 //    Records go to the list of their bucket (kBucketEntries consecutive rows
 //    of the level): ranks within the workgroup come from integer LDS atomics,
 //    room in each list from one global atomic per (workgroup, level, bucket).
@@ -483,11 +483,14 @@ constexpr int kBucketShift = 13;
 constexpr int kBucketEntries = 1 << kBucketShift;
 constexpr int kMaxBucketsPerLevel = 512;
 // Same-cell lane groups of kMinGroup or more lanes are summed across lanes
-// (measured on a full training run: a minimum of 2 or 3 costs more in
-// cross-lane sums than it saves in records, 4 to 6 measure alike); the search for groups (one leader per step, lowest pending lane
-// first) stops after kMaxSingles leaders in a row with a smaller group, and
-// is skipped on levels where no lane shares the cell of the lane before it
-// (the fine levels).
+// (group_sums; measured on the late iterations of a full training run, ms
+// per hash-grid backward: a minimum of 3 / 4 / 5 / 6 / 8 lanes -> 1.07 /
+// 1.05 / 1.036 / 1.03 / 1.045: smaller groups cost more in cross-lane sums
+// than they save in records); the search for groups (one leader per step,
+// lowest pending lane first) stops after kMaxSingles leaders in a row with a
+// smaller group (3 / 6 / 10 -> 1.05 / 1.03 / 1.044 ms), and is skipped on
+// levels where no lane shares the cell of the lane before it (the fine
+// levels).
 #ifndef SN_MIN_GROUP
 #define SN_MIN_GROUP 6
 #endif
