@@ -420,9 +420,6 @@ def main(argv=None):
     ap.add_argument("--resolutions", default="256,512,768,1024")
     ap.add_argument("--views", default="8,50")
     ap.add_argument("--one-at-a-time", action="store_true")
-    ap.add_argument("--both", action="store_true",
-                    help="run the stream twice on the same prepared inputs: with "
-                         "--max-running jobs at once, then one at a time")
     ap.add_argument("--no-fuse", action="store_true",
                     help="each job replays its own graphs on its own stream")
     a = ap.parse_args(argv)
@@ -441,19 +438,13 @@ def main(argv=None):
         data[sp.job_id] = job_data(sp, "cuda")
         print(json.dumps({"input_ready": sp.job_id, "s": round(time.perf_counter() - t, 2)}),
               flush=True)
-    runs = [a.max_running, 1] if a.both else [1 if a.one_at_a_time else a.max_running]
-    for k, max_running in enumerate(runs):
-        last = k == len(runs) - 1
-        # the inputs are read only: a second run on the same ones is the same stream
-        take = (lambda sp: data.pop(sp.job_id)) if last else (lambda sp: data[sp.job_id])
-        server = Server(max_running=max_running, precision="fp16", fused=not a.no_fuse,
-                        make_job=lambda sp, take=take: Job(sp, "cuda", "fp16", data=take(sp)))
-        out = server.run(specs, log=log)
-        out.pop("records")
-        out["max_running"] = max_running
-        print(json.dumps(out), flush=True)
-        del server
-        torch.cuda.empty_cache()
+    max_running = 1 if a.one_at_a_time else a.max_running
+    server = Server(max_running=max_running, precision="fp16", fused=not a.no_fuse,
+                    make_job=lambda sp: Job(sp, "cuda", "fp16", data=data.pop(sp.job_id)))
+    out = server.run(specs, log=log)
+    out.pop("records")
+    out["max_running"] = max_running
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

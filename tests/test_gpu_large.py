@@ -13,8 +13,8 @@
   * ray marching on a 256^3 grid bit-exact against the C oracle (8192 rays);
   * one iteration of the prims1024 workload on the device iteration
     (SyncFreeStep, fp32, no fp16 table copy) against FusedStep on the same
-    8192 patches: the same marched / visible counts, loss and gradients within
-    the order of fp32 atomics.
+    8192 patches: the same marched / visible counts, the loss within 1e-4 and
+    every parameter tensor's gradient within 1e-3 (relative 2-norm).
 """
 import numpy as np
 import pytest
@@ -160,8 +160,21 @@ def test_prims1024_iteration_matches_fused_step(dev):
     assert int(sf.n0) == res.n_marched and int(sf.ns) == res.n_samples
     loss = float(losses[0] + losses[1] + losses[2])
     assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    # per parameter tensor (hash table, each MLP weight_v / weight_g / bias, the
+    # variance): relative 2-norm of the difference <= 1e-3.  The two paths
+    # fold the weight norm with different reductions (torch's norm in the
+    # fused iteration, a wavefront sum in the device iteration), so their MLP
+    # weights differ by an ulp; a gradient element whose terms nearly cancel
+    # then moves by a few percent of itself (but below 1e-3 of the largest
+    # element of the flat gradient), while a wrong term moves the tensor's
+    # norm by O(1).
     scale = float(g_ref.abs().max())
-    # the MLP weights are packed by different kernels in the two paths (one-ulp
-    # differences of the weight-norm sums), so the bound is looser than the
-    # fixed-order sums alone would need
-    torch.testing.assert_close(flat.grad, g_ref, rtol=1e-3, atol=1e-4 * scale)
+    for prm, off in zip(flat.params, flat.offsets):
+        a = flat.grad[off:off + prm.numel()].double()
+        b = g_ref[off:off + prm.numel()].double()
+        rel = float((a - b).norm() / b.norm().clamp_min(1e-30))
+        worst = float((a - b).abs().max())
+        print(f"grad {tuple(prm.shape)}: |g| {float(b.norm()):.4e} rel {rel:.2e} "
+              f"max |diff| {worst:.2e}", flush=True)
+        assert rel <= 1e-3, (tuple(prm.shape), rel)
+        assert worst <= 1e-3 * scale, (tuple(prm.shape), worst)
