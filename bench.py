@@ -99,7 +99,7 @@ def hash_fwd_bytes(n, n_active, grid, table_bytes=4):
 # active), per point: two rocprofv3 counter runs over --traffic-probe (5 launches
 # on the last iteration's points), FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 -- the
 # factors are what a 64 MiB copy reads on each counter.
-HASH_BWD_TRAFFIC_PER_POINT = {"bytes_per_point": 2740882158 / 1473021, "probe_points": 1473021,
+HASH_BWD_TRAFFIC_PER_POINT = {"bytes_per_point": 2277261689 / 1229823, "probe_points": 1229823,
                               "active_levels": 16}
 
 
