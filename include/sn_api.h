@@ -597,8 +597,8 @@ int sn_hashgrid_dx_dev(const float* x, int64_t n_cap, const int32_t* n_dev, cons
 /* the bytes for records_per_bucket times the bucket count; -1: unsupported */
 /* config).  Each call shares the pool out by the counts the previous call  */
 /* on the same workspace left in it (5/4 of each list's count plus a floor, */
-/* at least the even split, at most 131072 records or the even split if    */
-/* larger, scaled to the pool; the                                          */
+/* at most 131072 records or the even split if larger; scaled down to the   */
+/* pool, or sharing what they leave of it evenly up to the same cap; the    */
 /* even split for a workspace no call has used): the                        */
 /* workspace carries this hint between calls, any content is valid input,  */
 /* and calls that share a workspace must not overlap.  Same sums as         */
@@ -621,6 +621,23 @@ int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                         const int32_t* n_active_dev,
                             const float* grad_enc, const float* g_gx, float* grad_grad_enc,
                             float* grad_params, float* grad_x, void* stream);
+/* The table gradient of the analytic-gradient iteration in one pass: the   */
+/* first-order term of grad_enc (as sn_hashgrid_bwd_ws_dev) plus the double */
+/* backward's table term that sn_hashgrid_bwd_bwd_dev adds for              */
+/* (grad_enc = grad_enc2, g_gx): per point, level and corner               */
+/* scale_l (g_gx . d w_c / d frac) grad_enc2[l, f].  Replaces tcnn's        */
+/* backward + backward_backward_input pair behind                           */
+/* autograd.grad(create_graph=True) (models/fields.py:107-119) for the     */
+/* table.  With a workspace the two terms share each corner's bucket record */
+/* (fixed-point sums, no fp32 atomic); without one (or F > 2) the second    */
+/* term is added with fp32 atomics.  Accumulates into grad_params.          */
+int sn_hashgrid_bwd_ad_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                              const float* params, int32_t n_levels, int32_t n_features,
+                              int32_t log2_hashmap_size, int32_t base_resolution,
+                              float per_level_scale, int32_t n_active_levels,
+                              const int32_t* n_active_dev, const float* grad_enc,
+                              const float* g_gx, const float* grad_enc2, float* grad_params,
+                              void* workspace, size_t workspace_bytes, void* stream);
 /* Cells of an occupancy update after warm-up (NA/nerfacc/grid.py:181-194): */
 /* n_cells/4 uniform cells then up to n_cells/4 occupied cells (all of them */
 /* in order when fewer), chosen from device uniforms.  *n_out on device.    */

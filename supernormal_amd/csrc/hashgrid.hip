@@ -673,12 +673,19 @@ struct StagedRecord {
   float v[2];
 };
 
-template <int F>
+// kAd: the records also carry the table's part of the double backward
+// through the analytic x-gradient (the term hash_bwd_bwd_kernel adds with
+// fp32 atomics): with the x-gradient's cotangent v = gv[i] and the cotangent
+// genc2 of the encoding's derivative along v, corner c of level l adds
+// scale_l (v . d w_c / d frac) genc2[i, l, f] to row idx_c, feature f.
+template <int F, bool kAd = false>
 __global__ void __launch_bounds__(kRecThreads)
 hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                     const float* __restrict__ params, HashGridDesc d,
                     const float* __restrict__ genc, float* __restrict__ gparams,
-                    float* __restrict__ gx, BucketDesc bd, BucketPool pool) {
+                    float* __restrict__ gx, BucketDesc bd, BucketPool pool,
+                    const float* __restrict__ gv = nullptr,
+                    const float* __restrict__ genc2 = nullptr) {
   __shared__ int32_t hist[kMaxBucketsPerLevel];   // records per bucket (this level)
   __shared__ int32_t lofs[kMaxBucketsPerLevel];   // their first staging slot
   __shared__ int32_t rroom[kMaxBucketsPerLevel];  // list room left at the workgroup's share
@@ -702,6 +709,11 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
     }
     const float* g = genc + (act ? i : 0) * (int64_t)(d.n_levels * F);
     float dx[3] = {0.f, 0.f, 0.f};
+    float v[3] = {0.f, 0.f, 0.f};
+    if (kAd && act) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[k] = gv[3 * i + k];
+    }
     // dL/denc of 4 levels per load when the rows allow 16-byte loads (F = 2):
     // a point's row is read in whole 32-byte pieces instead of 8 bytes per
     // level, which the L2 (4 MB per XCD against ~4 MB of rows in flight per
@@ -735,6 +747,21 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       for (int c = 0; c < 8; ++c)
 #pragma unroll
         for (int f = 0; f < F; ++f) rv[c][f] = w[c] * gl[f];
+      if constexpr (kAd) {
+        float g2[F];
+        const float* g2p = genc2 + (act ? i : 0) * (int64_t)(d.n_levels * F) + l * F;
+#pragma unroll
+        for (int f = 0; f < F; ++f) g2[f] = act ? g2p[f] : 0.f;
+        float dw[8][3];
+        hg_dweights(fr, dw);
+        const float sc = d.scale[l];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const float a = sc * (v[0] * dw[c][0] + v[1] * dw[c][1] + v[2] * dw[c][2]);
+#pragma unroll
+          for (int f = 0; f < F; ++f) rv[c][f] += a * g2[f];
+        }
+      }
       bool has = act;
       uint64_t pend = __ballot(act);
       {
@@ -757,7 +784,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 #pragma unroll
           for (int c = 0; c < 8; ++c)
 #pragma unroll
-            for (int f = 0; f < F; ++f) a[c * F + f] = in ? w[c] * gl[f] : 0.f;
+            for (int f = 0; f < F; ++f) a[c * F + f] = in ? rv[c][f] : 0.f;
           group_sums<F>(a, lane, ld, rv);
           if (in && lane != ld) has = false;
           singles = 0;
@@ -1230,24 +1257,18 @@ int64_t sn_hashgrid_bwd_workspace_bytes(int32_t n_levels, int32_t n_features,
   return (int64_t)bucket_ws_bytes(nb, n_features, total);
 }
 
-int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
-                           const float* params, int32_t n_levels, int32_t n_features,
-                           int32_t log2_hashmap_size, int32_t base_resolution,
-                           float per_level_scale, int32_t n_active_levels,
-                        const int32_t* n_active_dev, const float* grad_enc,
-                           float* grad_params, float* grad_x, void* workspace,
-                           size_t workspace_bytes, void* stream) {
-  clear_error();
-  HashGridDesc d;
-  int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
-                    base_resolution, per_level_scale, n_active_levels, &d);
-  if (rc) return rc;
-  d.n_active_dev = n_active_dev;
-  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd: n < 0");
-  if (n_cap == 0) return SN_OK;
-  SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
-  SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
-  hipStream_t s = as_stream(stream);
+}  // extern "C"
+
+// The backward to the table (and x): bucket records with a workspace (with
+// gv / genc2 also the double backward's table term), else the atomic paths;
+// returns 1 in *bucketed when the records carried gv / genc2.
+static int hashgrid_bwd_impl(const float* x, int64_t n_cap, const int32_t* n_dev,
+                             const float* params, HashGridDesc d, int32_t n_features,
+                             const float* grad_enc, const float* gv, const float* genc2,
+                             float* grad_params, float* grad_x, void* workspace,
+                             size_t workspace_bytes, hipStream_t s, int* bucketed) {
+  int rc = SN_OK;
+  *bucketed = 0;
   const unsigned grid = grid_for(n_cap, 256, kHashBlocks);
   if (g_aggregate == 2 && grad_params && n_features <= 2) {
     // bucket records when a workspace is given (and the 32-bit record
@@ -1281,12 +1302,23 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
       bucket_layout_kernel<<<1, kLayoutThreads, 0, s>>>(pool, (int32_t)nb);
       rc = check_launch("sn_hashgrid_bwd");
       if (rc) return rc;
-      if (n_features == 1)
-        hash_bwd_rec_kernel<1><<<grid_for(n_cap, kRecThreads, kHashBlocks), kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                    grad_params, grad_x, bd, pool);
+      const unsigned rgrid = grid_for(n_cap, kRecThreads, kHashBlocks);
+      const bool ad = gv && genc2;
+      if (n_features == 1 && ad)
+        hash_bwd_rec_kernel<1, true><<<rgrid, kRecThreads, 0, s>>>(
+            x, n_cap, n_dev, params, d, grad_enc, grad_params, grad_x, bd, pool, gv, genc2);
+      else if (n_features == 1)
+        hash_bwd_rec_kernel<1><<<rgrid, kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d,
+                                                             grad_enc, grad_params, grad_x, bd,
+                                                             pool);
+      else if (ad)
+        hash_bwd_rec_kernel<2, true><<<rgrid, kRecThreads, 0, s>>>(
+            x, n_cap, n_dev, params, d, grad_enc, grad_params, grad_x, bd, pool, gv, genc2);
       else
-        hash_bwd_rec_kernel<2><<<grid_for(n_cap, kRecThreads, kHashBlocks), kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
-                                                    grad_params, grad_x, bd, pool);
+        hash_bwd_rec_kernel<2><<<rgrid, kRecThreads, 0, s>>>(x, n_cap, n_dev, params, d,
+                                                             grad_enc, grad_params, grad_x, bd,
+                                                             pool);
+      *bucketed = ad ? 1 : 0;
       rc = check_launch("sn_hashgrid_bwd");
       if (rc) return rc;
       if (n_features == 1)
@@ -1309,6 +1341,62 @@ int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                 hash_bwd_kernel<kF><<<grid, 256, 0, s>>>(x, n_cap, n_dev, params, d, grad_enc,
                                                          grad_params, grad_x, agg));
   return check_launch("sn_hashgrid_bwd");
+}
+
+extern "C" {
+
+int sn_hashgrid_bwd_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                           const float* params, int32_t n_levels, int32_t n_features,
+                           int32_t log2_hashmap_size, int32_t base_resolution,
+                           float per_level_scale, int32_t n_active_levels,
+                           const int32_t* n_active_dev, const float* grad_enc,
+                           float* grad_params, float* grad_x, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_bwd", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  d.n_active_dev = n_active_dev;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && params && grad_enc, "sn_hashgrid_bwd: null input");
+  SN_REQUIRE(grad_params || grad_x, "sn_hashgrid_bwd: no output");
+  int bucketed = 0;
+  return hashgrid_bwd_impl(x, n_cap, n_dev, params, d, n_features, grad_enc, nullptr, nullptr,
+                           grad_params, grad_x, workspace, workspace_bytes, as_stream(stream),
+                           &bucketed);
+}
+
+int sn_hashgrid_bwd_ad_ws_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                              const float* params, int32_t n_levels, int32_t n_features,
+                              int32_t log2_hashmap_size, int32_t base_resolution,
+                              float per_level_scale, int32_t n_active_levels,
+                              const int32_t* n_active_dev, const float* grad_enc,
+                              const float* g_gx, const float* grad_enc2, float* grad_params,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_bwd_ad", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  d.n_active_dev = n_active_dev;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_bwd_ad: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && params && grad_enc && g_gx && grad_enc2 && grad_params,
+             "sn_hashgrid_bwd_ad: null argument");
+  hipStream_t s = as_stream(stream);
+  int bucketed = 0;
+  rc = hashgrid_bwd_impl(x, n_cap, n_dev, params, d, n_features, grad_enc, g_gx, grad_enc2,
+                         grad_params, nullptr, workspace, workspace_bytes, s, &bucketed);
+  if (rc || bucketed) return rc;
+  // no bucket path (no workspace, F > 2): the double backward's table term
+  // with fp32 atomics
+  SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256, 0,
+                                                      s>>>(x, n_cap, n_dev, params, d,
+                                                           grad_enc2, g_gx, nullptr,
+                                                           grad_params, nullptr));
+  return check_launch("sn_hashgrid_bwd_ad");
 }
 
 int sn_hashgrid_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev, const float* params,

@@ -174,10 +174,11 @@ class _SDFGradFn(torch.autograd.Function):
              ptr(g_mlp), 0, ptr(ws), ws.numel(), s)
         ge = g_h0[:, 3:].contiguous()
         ged = g_h0d[:, 3:].contiguous()
-        hashgrid_bwd(x, n, None, params, hargs, ge, g_params, None, s,
-                     ws=cfg.ws(ctx.n_active, n, dev))
-        call("sn_hashgrid_bwd_bwd", ptr(x), n, ptr(params), *hargs, ptr(ged), ptr(v), None,
-             ptr(g_params), None, s)
+        # the table: first-order term of ge and the double backward's term of
+        # (ged, v) in one pass (shared bucket records)
+        hws = cfg.ws(ctx.n_active, n, dev)
+        call("sn_hashgrid_bwd_ad_ws_dev", ptr(x), n, None, ptr(params), *hargs, None, ptr(ge),
+             ptr(v), ptr(ged), ptr(g_params), ptr(hws), 0 if hws is None else hws.numel(), s)
         return None, g_params, g_mlp, None, None
 
 

@@ -486,16 +486,19 @@ class SyncFreeStep:
 
     def _part_hash(self):
         """Hash-grid backward into the flat gradient buffer (bucket records,
-        then per-bucket sums); with 'ad' also the table's part of the double
-        backward (fp32 atomics)."""
+        then per-bucket sums); with 'ad' the records also carry the table's
+        part of the double backward (one pass, no fp32 atomics)."""
         params = self.net.encoding.params
         s = stream_of(self.dev)
+        if self.ad:
+            ws = self.ws_hash
+            call("sn_hashgrid_bwd_ad_ws_dev", ptr(self.pts), self.c1, ptr(self.ne), ptr(params),
+                 *self._hash_args(), ptr(self.levels_dev), ptr(self.g_enc), ptr(self.g_grad),
+                 ptr(self.g_enc_dot), ptr(params.grad), ptr(ws), 0 if ws is None else ws.numel(),
+                 s)
+            return
         hashgrid_bwd(self.pts, self.c1, self.ne, params, self._hash_args(), self.g_enc,
                      params.grad, None, s, ws=self.ws_hash, n_active_dev=self.levels_dev)
-        if self.ad:
-            call("sn_hashgrid_bwd_bwd_dev", ptr(self.pts), self.c1, ptr(self.ne), ptr(params),
-                 *self._hash_args(), ptr(self.levels_dev), ptr(self.g_enc_dot),
-                 ptr(self.g_grad), None, ptr(params.grad), None, s)
 
     def _part_guard(self):
         """Whether this iteration's optimiser step runs, and the device step

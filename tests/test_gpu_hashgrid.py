@@ -253,6 +253,52 @@ def test_double_backward_vs_fp64(dev):
     torch.testing.assert_close(dx.cpu().double(), rx, rtol=1e-4, atol=1e-3)
 
 
+@pytest.mark.parametrize("with_ws", [True, False])
+def test_ad_table_gradient_one_pass(dev, with_ws):
+    """sn_hashgrid_bwd_ad_ws_dev: the table gradient of
+    L = sum(genc * enc) + sum(v * dL2/dx), dL2/dx = autograd.grad(sum(genc2 * enc), x),
+    i.e. the first-order term and the double backward's table term, against
+    the fp64 oracle's autograd.  With a workspace both terms share each
+    corner's bucket record (repeat calls on the same workspace agree bit for
+    bit once the layout follows the counts); without one the second term
+    takes fp32 atomics."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=21)
+    d = _desc(cfg)
+    rng = np.random.default_rng(22)
+    n = 3000
+    x = (rng.random((n, 3)) * 1.8 - 0.9).astype(np.float32)
+    g = rng.normal(size=(n, 32)).astype(np.float32)
+    g2 = rng.normal(size=(n, 32)).astype(np.float32)
+    v = rng.normal(size=(n, 3)).astype(np.float32)
+    x64 = torch.from_numpy(x).double().requires_grad_(True)
+    p64 = e.params.detach().cpu().double().requires_grad_(True)
+    enc = encode(x64, p64, d)
+    gx2, = torch.autograd.grad((enc * torch.from_numpy(g2).double()).sum(), x64,
+                               create_graph=True)
+    L = (enc * torch.from_numpy(g).double()).sum() + (gx2 * torch.from_numpy(v).double()).sum()
+    ref, = torch.autograd.grad(L, p64)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    xt, gt, g2t, vt = t(x), t(g), t(g2), t(v)
+    args = (16, 2, 19, 16, 1.38, 16)
+    nbytes = _lib.lib().sn_hashgrid_bwd_workspace_bytes(*args[:5], 16384) if with_ws else 0
+    ws = torch.zeros(max(nbytes, 1), dtype=torch.uint8, device=dev)
+    outs = []
+    for _ in range(3):
+        gp = torch.zeros_like(e.params)
+        call("sn_hashgrid_bwd_ad_ws_dev", ptr(xt), n, None, ptr(e.params), *args, None, ptr(gt),
+             ptr(vt), ptr(g2t), ptr(gp), ptr(ws) if with_ws else None, nbytes, stream_of(dev))
+        torch.cuda.synchronize()
+        outs.append(gp)
+    scale = float(ref.abs().max())
+    for gp in outs:
+        torch.testing.assert_close(gp.cpu().double(), ref, rtol=1e-4, atol=1e-5 * scale)
+    if with_ws:
+        assert torch.equal(outs[1], outs[2])
+
+
 def test_full_size_properties(dev):
     """cfg2 table at the training batch size: partition of unity and linearity in
     the table (size-independent checks)."""
