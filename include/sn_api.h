@@ -376,6 +376,23 @@ int sn_render_step(int32_t n_patches, int32_t patch_h, int32_t patch_w, const fl
 int sn_mlp16_fwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
                  const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
                  float beta, float* sdf, void* stream);
+/* sn_mlp16_fwd plus d sdf / d h0 [n, 3 + enc_dims] (columns [x, enc]): the   */
+/* MLP's part of SDFNetwork.gradient (models/fields.py:107-119, the rendered  */
+/* normal of the analytic-gradient method, renderer.py:225-226), replacing    */
+/* autograd.grad of the torch MLP; the same fp16 weights backward, with the   */
+/* cotangents as hi + lo fp16 parts.  The table's part: sn_hashgrid_dx_dev.   */
+int sn_mlp16_fwd_grad(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                      const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                      float beta, float* sdf, float* dh0, void* stream);
+/* sn_mlp_bwd_ad_enc on the matrix cores (the double backward of the         */
+/* analytic gradient, fields.py:107-119 with create_graph=True): the forward  */
+/* and its tangent along (v, enc_dot) recomputed on fp16, the two reverse     */
+/* passes on bf16 with fp32 accumulation.  Workspace as sn_mlp16_bwd.         */
+int sn_mlp16_bwd_ad_enc(const float* x, const float* enc, const float* v, const float* enc_dot,
+                        int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                        int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                        float* g_enc, float* g_enc_dot, float* g_mlp, int32_t accumulate,
+                        void* workspace, size_t workspace_bytes, void* stream);
 size_t sn_mlp16_bwd_workspace_bytes(int64_t n_cap, int32_t enc_dims, int32_t d_hidden,
                                     int32_t n_layers);
 int sn_mlp16_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,

@@ -61,12 +61,15 @@ SIGNATURES = {
                                 _p, _p, _i32, _p, _sz, _p]),
     "sn_mlp_bwd_ad_enc": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p,
                                     _p, _p, _p, _i32, _p, _sz, _p]),
+    "sn_mlp16_bwd_ad_enc": (C.c_int, [_p, _p, _p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p,
+                                      _p, _p, _p, _i32, _p, _sz, _p]),
     "sn_render_step_waves": (C.c_int, [_i32, _i32]),
     "sn_render_step": (C.c_int, [_i32, _i32, _i32] + [_p] * 16 + [_f32, _f32, _f32, _i32]
                        + [_p] * 8),
     "sn_adam_step": (C.c_int, [_p, _p, _p, _p, _i64, _f32, _p, C.c_double, C.c_double,
                                C.c_double, _i32, _p, _p]),
     "sn_mlp16_fwd": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p]),
+    "sn_mlp16_fwd_grad": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p, _p]),
     "sn_mlp16_bwd_workspace_bytes": (_sz, [_i64, _i32, _i32, _i32]),
     "sn_mlp16_bwd": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _i32, _i32, _f32, _p, _p, _p, _p,
                                _i32, _p, _sz, _p]),

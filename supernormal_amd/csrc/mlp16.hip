@@ -108,6 +108,12 @@ struct Cfg {
   static constexpr size_t tile_bytes = 2 * 64 * (16 * kWaves + 4) * 2;
   static constexpr size_t bwd_lds =
       (size_t)(nW0 + nW1 + nW1T + nW0T) * 64 * 16 + kVecFloats * 4 + tile_bytes;
+  // forward with the gradient d sdf / d h0: all fragments, no tiles
+  static constexpr size_t fwdg_lds =
+      (size_t)(nW0 + nW1 + nW1T + nW0T) * 64 * 16 + kVecFloats * 4;
+  // forward-over-reverse backward: four block tiles (cotangents and tangents)
+  static constexpr size_t bwdad_lds =
+      (size_t)(nW0 + nW1 + nW1T + nW0T) * 64 * 16 + kVecFloats * 4 + 2 * tile_bytes;
   __host__ __device__ static int n_params(int d) {
     return H * d + H + (NH == 2 ? H * H + H : 0) + H + 1;
   }
@@ -124,15 +130,17 @@ __device__ __forceinline__ int h0_col(int c, int E) {
   return c < E ? 3 + c : (c < E + 3 ? c - E : (c == E + 3 ? -2 : -1));
 }
 
-// Builds the A-operand fragments and bias vectors in LDS.
-template <int H, int NH, int KB0, bool kBwd>
+// Builds the A-operand fragments and bias vectors in LDS; with kBwd also the
+// transposed weights (for products W^T g), in type TT (bf16 for the training
+// backward, fp16 for the forward's gradient).
+template <int H, int NH, int KB0, bool kBwd, typename TT = __bf16>
 __device__ void build_frags(char* smem, const float* __restrict__ mlp, int E) {
   using C = Cfg<H, NH, KB0>;
   const int d = 3 + E;
   _Float16* fW0 = reinterpret_cast<_Float16*>(smem);
   _Float16* fW1 = fW0 + C::nW0 * 512;
-  __bf16* fW1T = reinterpret_cast<__bf16*>(fW1 + C::nW1 * 512);
-  __bf16* fW0T = fW1T + (kBwd ? C::nW1T * 512 : 0);
+  TT* fW1T = reinterpret_cast<TT*>(fW1 + C::nW1 * 512);
+  TT* fW0T = fW1T + (kBwd ? C::nW1T * 512 : 0);
   float* vec = reinterpret_cast<float*>(fW0T + (kBwd ? C::nW0T * 512 : 0));
   const float* W0 = mlp;
   const float* W1 = mlp + C::gW1(d);
@@ -152,13 +160,13 @@ __device__ void build_frags(char* smem, const float* __restrict__ mlp, int E) {
     for (int t = threadIdx.x; t < C::nW1T * 512; t += blockDim.x) {
       const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
       const int mi = f / C::KBH, kb = f % C::KBH;
-      fW1T[t] = (__bf16)W1[hid_unit(kb, q, j) * H + 16 * mi + r];
+      fW1T[t] = (TT)W1[hid_unit(kb, q, j) * H + 16 * mi + r];
     }
     for (int t = threadIdx.x; t < C::nW0T * 512; t += blockDim.x) {
       const int f = t / 512, e = t % 512, lane = e >> 3, j = e & 7, r = lane & 15, q = lane >> 4;
       const int md = f / C::KBH, kb = f % C::KBH;
       const int col = h0_col(16 * md + r, E);
-      fW0T[t] = (__bf16)(col >= 0 ? W0[hid_unit(kb, q, j) * d + col] : 0.f);
+      fW0T[t] = (TT)(col >= 0 ? W0[hid_unit(kb, q, j) * d + col] : 0.f);
     }
   }
   for (int t = threadIdx.x; t < H; t += blockDim.x) {
@@ -173,9 +181,10 @@ __device__ void build_frags(char* smem, const float* __restrict__ mlp, int E) {
 // vec4 (E % 4 == 0, 16-byte rows) the slots come in two 4-dim groups, each
 // either 4 encoding dims (one 16-byte load), the group [x, 1] that follows
 // them, or padding; otherwise one guarded load per slot.
+// `one` is the value of the constant slot: 1 for h0, 0 for a tangent of h0.
 __device__ __forceinline__ void load_h0(const float* __restrict__ x, const float* __restrict__ enc,
                                         int E, bool vec4, int64_t p, bool ok, int kb, int q,
-                                        float (&v)[8]) {
+                                        float (&v)[8], float one = 1.f) {
   const int c0 = 32 * kb + 8 * q;
   if (vec4) {
 #pragma unroll
@@ -185,7 +194,7 @@ __device__ __forceinline__ void load_h0(const float* __restrict__ x, const float
       if (ok && c4 < E) {
         a = *reinterpret_cast<const float4*>(enc + p * E + c4);
       } else if (ok && c4 == E) {
-        a = float4{x[p * 3], x[p * 3 + 1], x[p * 3 + 2], 1.f};
+        a = float4{x[p * 3], x[p * 3 + 1], x[p * 3 + 2], one};
       }
       v[4 * h] = a.x;
       v[4 * h + 1] = a.y;
@@ -198,7 +207,7 @@ __device__ __forceinline__ void load_h0(const float* __restrict__ x, const float
   for (int j = 0; j < 8; ++j) {
     const int c = c0 + j;
     const float* src = c < E ? enc + p * E + c : x + p * 3 + (c - E);
-    float t = (ok && c == E + 3) ? 1.f : 0.f;
+    float t = (ok && c == E + 3) ? one : 0.f;
     if (ok && c < E + 3) t = *src;
     v[j] = t;
   }
@@ -358,6 +367,97 @@ mlp16_fwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// forward with the analytic gradient: sdf [n] and d sdf / d h0 [n, 3 + E]
+// (packed column order [x, enc], the fp32 kernel's), for the rendered normal
+// grad_x sdf = dsdf/dx + sum_l dsdf/denc_l denc_l/dx (fields.py:107-119).  The
+// forward as in mlp16_fwd_kernel (fp16 weights, hi + lo activations); the
+// backprop of 1 through the same fp16 weights (transposed fragments), with
+// the cotangents split into hi + lo fp16 parts, so the gradient is that of
+// the network the sdf value comes from to the same resolution.
+// ---------------------------------------------------------------------------
+template <int H, int NH, int KB0>
+__global__ void __launch_bounds__(64 * kWaves)
+mlp16_fwd_grad_kernel(const float* __restrict__ x, const float* __restrict__ enc, int E, int vec4,
+                      int64_t n_cap, const int32_t* __restrict__ n_dev,
+                      const float* __restrict__ mlp, float beta, float* __restrict__ sdf_out,
+                      float* __restrict__ dh0) {
+  using C = Cfg<H, NH, KB0>;
+  constexpr int MB = C::MB, KBH = C::KBH, MD0 = C::MD0;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  build_frags<H, NH, KB0, true, _Float16>(smem, mlp, E);
+  __syncthreads();
+  const half8* fW0 = reinterpret_cast<const half8*>(smem);
+  const half8* fW1 = fW0 + C::nW0 * 64;
+  const half8* fW1T = fW1 + C::nW1 * 64;
+  const half8* fW0T = fW1T + C::nW1T * 64;
+  const float* vec = reinterpret_cast<const float*>(fW0T + C::nW0T * 64);
+  const float inv_beta = 1.f / beta;
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  const int d = 3 + E;
+  const int md_used = (E + 4 + 15) / 16;
+  const int64_t n_groups = ceil_div(n, 16);
+  const int64_t waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t g = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6); g < n_groups; g += waves) {
+    asm volatile("" ::: "memory");  // fragments re-read from LDS each group
+    const int64_t p = g * 16 + r;
+    const bool ok = p < n;
+    int qs = q, Es = E, v4s = vec4;
+    asm volatile("" : "+v"(qs), "+s"(Es), "+s"(v4s));
+    float h0[KB0][8];
+#pragma unroll
+    for (int kb = 0; kb < KB0; ++kb) load_h0(x, enc, Es, v4s, p, ok, kb, qs, h0[kb]);
+    Fwd<H, NH, KB0> F;
+    float s = 0.f, gz[MB][4];
+    forward_group<H, NH, KB0, true, true>(
+        fW0, fW1, vec, h0, beta, inv_beta, F, [&](int mb, int e, float h, float dd) {
+          const float wo = vec[2 * H + 16 * mb + 4 * q + e];
+          s += wo * h;
+          gz[mb][e] = wo * dd;  // d sdf / d z of the last hidden layer
+        });
+    s = sum_q(s) + vec[3 * H];
+    if (q == 0 && ok) sdf_out[p] = s;
+    if (NH == 2) {
+      half8 gh[KBH], gl[KBH];
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) hid_operand_h<MB>(gz, kb, gh[kb], gl[kb]);
+#pragma unroll
+      for (int mi = 0; mi < MB; ++mi) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const half8 w = fW1T[(mi * KBH + kb) * 64 + l];
+          acc = mma_h(w, gl[kb], acc);
+          acc = mma_h(w, gh[kb], acc);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) gz[mi][e] = acc[e] * F.d1[mi][e];
+      }
+    }
+    half8 gh[KBH], gl[KBH];
+#pragma unroll
+    for (int kb = 0; kb < KBH; ++kb) hid_operand_h<MB>(gz, kb, gh[kb], gl[kb]);
+#pragma unroll
+    for (int md = 0; md < MD0; ++md) {
+      if (md >= md_used) break;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) {
+        const half8 w = fW0T[(md * KBH + kb) * 64 + l];
+        acc = mma_h(w, gl[kb], acc);
+        acc = mma_h(w, gh[kb], acc);
+      }
+      // internal dims 16 md + 4q + e of point p, in the packed column order
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = h0_col(16 * md + 4 * qs + e, Es);
+        if (ok && col >= 0) dh0[p * d + col] = acc[e];
+      }
+    }
+  }
+}
+
 // Block tiles: rows of bf16 over the 64 points of a block step (wavefront w
 // owns columns 16w..16w+15).  Row pitch 68 bf16 (34 dwords): the writes of a
 // wavefront (4 rows 4 apart, 16 columns each) fall on 32 distinct banks, and
@@ -389,6 +489,76 @@ __device__ __forceinline__ floatx4 tile_mma(const __bf16* tA, int a, const __bf1
   for (int k = 0; k < kWaves; ++k)
     acc = mma_b4(tile_operand(tA, 16 * a + r, k), tile_operand(tB, 16 * b + r, k), acc);
   return acc;
+}
+
+// The workgroup's partial row of the weight gradient (packed layout): weight
+// blocks from the wavefronts that own them, bias sums combined over the
+// wavefronts in a fixed order through LDS (red: >= (2 kWaves H + kWaves) floats
+// of LDS no longer in use).  Called by every lane of the workgroup.
+template <int H, int NH, int KB0, int T1, int T0>
+__device__ __forceinline__ void write_partial16(float* __restrict__ out, int d, int E, int wid,
+                                                float* red, const floatx4 (&acc1)[T1],
+                                                const floatx4 (&acc0)[T0],
+                                                const float (&db1)[H / 16][4],
+                                                const float (&dwo)[H / 16][4], float dbo) {
+  using C = Cfg<H, NH, KB0>;
+  constexpr int MB = C::MB, MD0 = C::MD0;
+  constexpr int NB1 = NH == 2 ? MB * MB : 0, NB0 = MB * MD0;
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  // the workgroup's partial row: weight blocks from their owners, bias sums
+  // combined over the wavefronts in a fixed order through LDS
+#pragma unroll
+  for (int t = 0; t < T0; ++t) {
+    const int b = wid + kWaves * t;
+    if (b >= NB0) continue;
+    const int ob = b / MD0, md = b % MD0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int col = h0_col(16 * md + r, E), o = 16 * ob + 4 * q + e;
+      if (col >= 0) out[o * d + col] = acc0[t][e];
+      else if (col == -2) out[C::gB0(d) + o] = acc0[t][e];
+    }
+  }
+  if (NH == 2) {
+#pragma unroll
+    for (int t = 0; t < T1; ++t) {
+      const int b = wid + kWaves * t;
+      if (b >= NB1) continue;
+      const int ob = b / MB, ib = b % MB;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        out[C::gW1(d) + (16 * ob + 4 * q + e) * H + 16 * ib + r] = acc1[t][e];
+    }
+  }
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = 16 * mb + 4 * q + e;
+      const float s1 = sum_r(db1[mb][e]), so = sum_r(dwo[mb][e]);
+      if (r == 0) {
+        red[(wid * 2 + 0) * H + u] = s1;
+        red[(wid * 2 + 1) * H + u] = so;
+      }
+    }
+  float sbo = sum_r(dbo);
+  sbo += __shfl_xor(sbo, 16, 64);
+  sbo += __shfl_xor(sbo, 32, 64);
+  if (l == 0) red[kWaves * 2 * H + wid] = sbo;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 2 * H + 1; t += blockDim.x) {
+    float v = 0.f;
+    if (t < 2 * H) {
+      for (int w = 0; w < kWaves; ++w) v += red[w * 2 * H + t];
+      const int k = t / H, u = t % H;
+      if (k == 0) { if (NH == 2) out[C::gB1(d) + u] = v; }
+      else out[C::gWo(d) + u] = v;
+    } else {
+      for (int w = 0; w < kWaves; ++w) v += red[kWaves * 2 * H + w];
+      out[C::gBo(d)] = v;
+    }
+  }
+
 }
 
 // ---------------------------------------------------------------------------
@@ -543,61 +713,262 @@ mlp16_bwd_kernel(const float* __restrict__ x, const float* __restrict__ enc, int
     __syncthreads();  // tiles are rewritten by the next step
   }
 
-  // the workgroup's partial row: weight blocks from their owners, bias sums
-  // combined over the wavefronts in a fixed order through LDS
-  float* out = partials + (int64_t)blockIdx.x * C::n_params(d);
+  write_partial16<H, NH, KB0>(partials + (int64_t)blockIdx.x * C::n_params(d), d, E, wid,
+                              reinterpret_cast<float*>(tileA), acc1, acc0, db1, dwo, dbo);
+}
+
+// ---------------------------------------------------------------------------
+// forward-over-reverse backward of J = sum_p a_p sdf(x_p) + v_p . grad_x sdf(x_p)
+// (the analytic gradient trained through its value, fields.py:107-119 with
+// create_graph), the bf16 counterpart of mlp_bwd_ad_kernel: tangents along v
+// of the inputs hd0 = [encd | v | 0] (encd: the encoding's directional
+// derivative, no bias slot) propagated with the forward, then the reverse
+// pass of the value (gb) and of the tangent (gt):
+//   last layer: gb = a wo s' + wo zd s'',  gt = wo s'   (s'' = beta s'(1 - s'))
+//   hidden:     gb_z = (W^T gb) s' + (W^T gt) zd s'',  gt_z = (W^T gt) s'
+//   weights:    dW += gb (x) h + gt (x) hd, db += gb, dwo += a h + s' zd, dbo += a
+// Outputs dJ/denc and dJ/dencd [n, E] (the encoding's columns) and the
+// weight-gradient partial rows.  Same structure as mlp16_bwd_kernel: the
+// forward recomputed from hi parts on fp16, cotangents on bf16; the weight
+// gradients through four block tiles (gb with h, gt with hd, summed in one
+// accumulator per block).
+// ---------------------------------------------------------------------------
+template <int H, int NH, int KB0>
+__global__ void __launch_bounds__(64 * kWaves, 1)
+mlp16_bwd_ad_kernel(const float* __restrict__ x, const float* __restrict__ enc,
+                    const float* __restrict__ v, const float* __restrict__ encd, int E, int vec4,
+                    int64_t n_cap, const int32_t* __restrict__ n_dev,
+                    const float* __restrict__ mlp, float beta, const float* __restrict__ a_sdf,
+                    float* __restrict__ g_enc, float* __restrict__ g_encd,
+                    float* __restrict__ partials) {
+  using C = Cfg<H, NH, KB0>;
+  constexpr int MB = C::MB, KBH = C::KBH, MD0 = C::MD0;
+  constexpr int NB1 = NH == 2 ? MB * MB : 0, NB0 = MB * MD0;
+  constexpr int T1 = NH == 2 ? (NB1 + kWaves - 1) / kWaves : 1, T0 = (NB0 + kWaves - 1) / kWaves;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  build_frags<H, NH, KB0, true>(smem, mlp, E);
+  __syncthreads();
+  const half8* fW0 = reinterpret_cast<const half8*>(smem);
+  const half8* fW1 = fW0 + C::nW0 * 64;
+  const bf16x8* fW1T = reinterpret_cast<const bf16x8*>(fW1 + C::nW1 * 64);
+  const bf16x8* fW0T = fW1T + C::nW1T * 64;
+  const float* vec = reinterpret_cast<const float*>(fW0T + C::nW0T * 64);
+  __bf16* tA = reinterpret_cast<__bf16*>(smem + C::bwdad_lds - 2 * C::tile_bytes);
+  __bf16* tB = tA + 64 * kTileP;
+  __bf16* tC = tB + 64 * kTileP;
+  __bf16* tD = tC + 64 * kTileP;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const float inv_beta = 1.f / beta;
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  const int d = 3 + E;
+  const int md_used = (E + 4 + 15) / 16;
+
+  floatx4 acc1[T1], acc0[T0];
 #pragma unroll
-  for (int t = 0; t < T0; ++t) {
-    const int b = wid + kWaves * t;
-    if (b >= NB0) continue;
-    const int ob = b / MD0, md = b % MD0;
+  for (int t = 0; t < T1; ++t) acc1[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int col = h0_col(16 * md + r, E), o = 16 * ob + 4 * q + e;
-      if (col >= 0) out[o * d + col] = acc0[t][e];
-      else if (col == -2) out[C::gB0(d) + o] = acc0[t][e];
+  for (int t = 0; t < T0; ++t) acc0[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float db1[MB][4], dwo[MB][4], dbo = 0.f;
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) db1[a][e] = dwo[a][e] = 0.f;
+
+  const int64_t n_steps = ceil_div(n, (int64_t)kBlkPts);
+  for (int64_t step = blockIdx.x; step < n_steps; step += gridDim.x) {
+    const int64_t p = (step * kWaves + wid) * 16 + r;
+    const bool ok = p < n;
+    int qs = q, Es = E, v4s = vec4;
+    asm volatile("" : "+v"(qs), "+s"(Es), "+s"(v4s));
+    const float as = ok ? a_sdf[p] : 0.f;
+    if (q == 0) dbo += as;
+    // layer 0 and its tangent (fp16, hi parts)
+    half8 bh[KB0], bdh[KB0];
+    {
+      float h0[KB0][8], hd0[KB0][8];
+#pragma unroll
+      for (int kb = 0; kb < KB0; ++kb) {
+        load_h0(x, enc, Es, v4s, p, ok, kb, qs, h0[kb]);
+        load_h0(v, encd, Es, v4s, p, ok, kb, qs, hd0[kb], 0.f);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB0; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          bh[kb][j] = (_Float16)h0[kb][j];
+          bdh[kb][j] = (_Float16)hd0[kb][j];
+        }
     }
-  }
-  if (NH == 2) {
+    float d1[MB][4], zd0[MB][4];  // s'(z0), tangent of z0 (NH == 2)
+    float gb[MB][4], gt[MB][4];   // cotangents of the current layer's pre-activation
+    if constexpr (NH == 2) {
+      float h1[MB][4], hd1[MB][4];
 #pragma unroll
-    for (int t = 0; t < T1; ++t) {
-      const int b = wid + kWaves * t;
-      if (b >= NB1) continue;
-      const int ob = b / MB, ib = b % MB;
+      for (int mb = 0; mb < MB; ++mb) {
+        floatx4 z = {0.f, 0.f, 0.f, 0.f}, zd = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        out[C::gW1(d) + (16 * ob + 4 * q + e) * H + 16 * ib + r] = acc1[t][e];
-    }
-  }
-  float* red = reinterpret_cast<float*>(tileA);  // [wave][2][H] + [wave] for bo
+        for (int kb = 0; kb < KB0; ++kb) {
+          const half8 w = fW0[(mb * KB0 + kb) * 64 + l];
+          z = mma_h(w, bh[kb], z);
+          zd = mma_h(w, bdh[kb], zd);
+        }
 #pragma unroll
-  for (int mb = 0; mb < MB; ++mb)
+        for (int e = 0; e < 4; ++e) {
+          const Sp sp = softplus(z[e], beta, inv_beta);
+          h1[mb][e] = sp.h;
+          d1[mb][e] = sp.d;
+          zd0[mb][e] = zd[e];
+          hd1[mb][e] = sp.d * zd[e];
+        }
+      }
+      // layer 1 and its tangent; the activations go to their tiles now
+      tile_store<MB>(tB, h1);
+      tile_store<MB>(tD, hd1);
+      half8 ch[KBH], cl[KBH], cdh[KBH], cdl[KBH];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int u = 16 * mb + 4 * q + e;
-      const float s1 = sum_r(db1[mb][e]), so = sum_r(dwo[mb][e]);
-      if (r == 0) {
-        red[(wid * 2 + 0) * H + u] = s1;
-        red[(wid * 2 + 1) * H + u] = so;
+      for (int kb = 0; kb < KBH; ++kb) {
+        hid_operand_h<MB>(h1, kb, ch[kb], cl[kb]);
+        hid_operand_h<MB>(hd1, kb, cdh[kb], cdl[kb]);
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        floatx4 z = {0.f, 0.f, 0.f, 0.f}, zd = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const half8 w = fW1[(mb * KBH + kb) * 64 + l];
+          z = mma_h(w, ch[kb], z);
+          zd = mma_h(w, cdh[kb], zd);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int u = 16 * mb + 4 * q + e;
+          const Sp sp = softplus(z[e] + vec[H + u], beta, inv_beta);
+          const float s2 = beta * sp.d * (1.f - sp.d), wo = vec[2 * H + u];
+          dwo[mb][e] += as * sp.h + sp.d * zd[e];
+          gb[mb][e] = as * wo * sp.d + wo * zd[e] * s2;
+          gt[mb][e] = wo * sp.d;
+          db1[mb][e] += gb[mb][e];
+        }
+      }
+      tile_store<MB>(tA, gb);
+      tile_store<MB>(tC, gt);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < T1; ++t) {
+        const int b = wid + kWaves * t;
+        if (b < NB1)
+          acc1[t] = tile_mma(tC, b / MB, tD, b % MB, tile_mma(tA, b / MB, tB, b % MB, acc1[t]));
+      }
+      // to layer 0 (bf16)
+      bf16x8 gbh[KBH], gbl[KBH], gth[KBH], gtl[KBH];
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) {
+        hid_operand_b<MB>(gb, kb, gbh[kb], gbl[kb]);
+        hid_operand_b<MB>(gt, kb, gth[kb], gtl[kb]);
+      }
+#pragma unroll
+      for (int mi = 0; mi < MB; ++mi) {
+        floatx4 ab = {0.f, 0.f, 0.f, 0.f}, at = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const bf16x8 w = fW1T[(mi * KBH + kb) * 64 + l];
+          ab = mma_b(w, gbh[kb], ab);
+          at = mma_b(w, gth[kb], at);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float s1 = d1[mi][e], s2 = beta * s1 * (1.f - s1);
+          gb[mi][e] = ab[e] * s1 + at[e] * zd0[mi][e] * s2;
+          gt[mi][e] = at[e] * s1;
+        }
+      }
+      __syncthreads();  // every wavefront is done reading the tiles
+    } else {
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        floatx4 z = {0.f, 0.f, 0.f, 0.f}, zd = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KB0; ++kb) {
+          const half8 w = fW0[(mb * KB0 + kb) * 64 + l];
+          z = mma_h(w, bh[kb], z);
+          zd = mma_h(w, bdh[kb], zd);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int u = 16 * mb + 4 * q + e;
+          const Sp sp = softplus(z[e], beta, inv_beta);
+          const float s2 = beta * sp.d * (1.f - sp.d), wo = vec[2 * H + u];
+          dwo[mb][e] += as * sp.h + sp.d * zd[e];
+          gb[mb][e] = as * wo * sp.d + wo * zd[e] * s2;
+          gt[mb][e] = wo * sp.d;
+        }
       }
     }
-  float sbo = sum_r(dbo);
-  sbo += __shfl_xor(sbo, 16, 64);
-  sbo += __shfl_xor(sbo, 32, 64);
-  if (l == 0) red[kWaves * 2 * H + wid] = sbo;
-  __syncthreads();
-  for (int t = threadIdx.x; t < 2 * H + 1; t += blockDim.x) {
-    float v = 0.f;
-    if (t < 2 * H) {
-      for (int w = 0; w < kWaves; ++w) v += red[w * 2 * H + t];
-      const int k = t / H, u = t % H;
-      if (k == 0) { if (NH == 2) out[C::gB1(d) + u] = v; }
-      else out[C::gWo(d) + u] = v;
-    } else {
-      for (int w = 0; w < kWaves; ++w) v += red[kWaves * 2 * H + w];
-      out[C::gBo(d)] = v;
+    // layer 0's weights: gb (x) h0 + gt (x) hd0 (the inputs read again)
+    tile_store<MB>(tA, gb);
+    tile_store<MB>(tC, gt);
+    {
+      float h0[KB0][8], hd0[KB0][8];
+#pragma unroll
+      for (int kb = 0; kb < KB0; ++kb) {
+        load_h0(x, enc, Es, v4s, p, ok, kb, qs, h0[kb]);
+        load_h0(v, encd, Es, v4s, p, ok, kb, qs, hd0[kb], 0.f);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB0; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          tB[(32 * kb + 8 * q + j) * kTileP + 16 * wid + r] = (__bf16)h0[kb][j];
+          tD[(32 * kb + 8 * q + j) * kTileP + 16 * wid + r] = (__bf16)hd0[kb][j];
+        }
     }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < T0; ++t) {
+      const int b = wid + kWaves * t;
+      if (b < NB0 && (b % MD0) < md_used)
+        acc0[t] = tile_mma(tC, b / MD0, tD, b % MD0, tile_mma(tA, b / MD0, tB, b % MD0, acc0[t]));
+    }
+    // dJ/denc = W0^T gb, dJ/dencd = W0^T gt (the encoding's columns)
+    {
+      bf16x8 gbh[KBH], gbl[KBH], gth[KBH], gtl[KBH];
+#pragma unroll
+      for (int kb = 0; kb < KBH; ++kb) {
+        hid_operand_b<MB>(gb, kb, gbh[kb], gbl[kb]);
+        hid_operand_b<MB>(gt, kb, gth[kb], gtl[kb]);
+      }
+#pragma unroll
+      for (int md = 0; md < MD0; ++md) {
+        if (md >= md_used) break;
+        floatx4 ab = {0.f, 0.f, 0.f, 0.f}, at = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+          const bf16x8 w = fW0T[(md * KBH + kb) * 64 + l];
+          ab = mma_b(w, gbh[kb], ab);
+          at = mma_b(w, gth[kb], at);
+        }
+        const int c0 = 16 * md + 4 * qs;  // internal dims c0..c0+3 of point p
+        if (v4s) {
+          if (ok && c0 < Es) {
+            *reinterpret_cast<float4*>(g_enc + p * Es + c0) = float4{ab[0], ab[1], ab[2], ab[3]};
+            *reinterpret_cast<float4*>(g_encd + p * Es + c0) = float4{at[0], at[1], at[2], at[3]};
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (ok && c0 + e < Es) {
+              g_enc[p * Es + c0 + e] = ab[e];
+              g_encd[p * Es + c0 + e] = at[e];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // tiles are rewritten by the next step
   }
+  write_partial16<H, NH, KB0>(partials + (int64_t)blockIdx.x * C::n_params(d), d, E, wid,
+                              reinterpret_cast<float*>(tA), acc1, acc0, db1, dwo, dbo);
 }
 
 // Raises a kernel's dynamic LDS limit above the default 64 KB (gfx950 has
@@ -676,6 +1047,27 @@ int sn_mlp16_fwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_c
   return check_launch("sn_mlp16_fwd");
 }
 
+int sn_mlp16_fwd_grad(const float* x, const float* enc, int32_t enc_dims, int64_t n_cap,
+                      const int32_t* n_dev, const float* mlp, int32_t d_hidden, int32_t n_layers,
+                      float beta, float* sdf, float* dh0, void* stream) {
+  clear_error();
+  int kb0 = 0;
+  int rc = check_cfg("sn_mlp16_fwd_grad", enc_dims, d_hidden, n_layers, &kb0);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0 && beta > 0.f, "sn_mlp16_fwd_grad: bad n / beta");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && mlp && sdf && dh0 && (enc || enc_dims == 0), "sn_mlp16_fwd_grad: null pointer");
+  const unsigned grid = blocks_for(n_cap, kFwdBlocksMax);
+  hipStream_t s = as_stream(stream);
+  SN_MLP16_DISPATCH(d_hidden, n_layers, kb0, {
+    constexpr size_t lds = Cfg<kH, kNH, kKB>::fwdg_lds;
+    allow_lds(mlp16_fwd_grad_kernel<kH, kNH, kKB>, lds);
+    mlp16_fwd_grad_kernel<kH, kNH, kKB><<<grid, 64 * kWaves, lds, s>>>(
+        x, enc, enc_dims, vec4_ok(enc, enc_dims), n_cap, n_dev, mlp, beta, sdf, dh0);
+  });
+  return check_launch("sn_mlp16_fwd_grad");
+}
+
 size_t sn_mlp16_bwd_workspace_bytes(int64_t n_cap, int32_t enc_dims, int32_t d_hidden,
                                     int32_t n_layers) {
   const int np = sn_mlp_n_params(3 + enc_dims, d_hidden, n_layers);
@@ -715,6 +1107,45 @@ int sn_mlp16_bwd(const float* x, const float* enc, int32_t enc_dims, int64_t n_c
         beta, g_sdf, g_enc, g_x, part);
   });
   if ((rc = check_launch("sn_mlp16_bwd"))) return rc;
+  return mlp_reduce_rows(part, (int64_t)grid, np, g_mlp, accumulate, s);
+}
+
+int sn_mlp16_bwd_ad_enc(const float* x, const float* enc, const float* v, const float* enc_dot,
+                        int32_t enc_dims, int64_t n_cap, const int32_t* n_dev, const float* mlp,
+                        int32_t d_hidden, int32_t n_layers, float beta, const float* g_sdf,
+                        float* g_enc, float* g_enc_dot, float* g_mlp, int32_t accumulate,
+                        void* ws, size_t ws_bytes, void* stream) {
+  clear_error();
+  int kb0 = 0;
+  int rc = check_cfg("sn_mlp16_bwd_ad_enc", enc_dims, d_hidden, n_layers, &kb0);
+  if (rc) return rc;
+  SN_REQUIRE(n_cap >= 0 && beta > 0.f, "sn_mlp16_bwd_ad_enc: bad n / beta");
+  SN_REQUIRE(mlp && g_mlp, "sn_mlp16_bwd_ad_enc: null pointer");
+  const int np = sn_mlp_n_params(3 + enc_dims, d_hidden, n_layers);
+  hipStream_t s = as_stream(stream);
+  if (n_cap == 0) {
+    if (accumulate) return SN_OK;
+    return fill_f32(g_mlp, np, 0.f, s);
+  }
+  SN_REQUIRE(x && v && g_sdf && g_enc && g_enc_dot && ((enc && enc_dot) || enc_dims == 0),
+             "sn_mlp16_bwd_ad_enc: null input");
+  const size_t need = sn_mlp16_bwd_workspace_bytes(n_cap, enc_dims, d_hidden, n_layers);
+  if (ws_bytes < need) {
+    set_error("sn_mlp16_bwd_ad_enc: workspace %zu < %zu bytes", ws_bytes, need);
+    return SN_ESPACE;
+  }
+  const unsigned grid = blocks_for(n_cap, kBwdBlocksMax);
+  float* part = reinterpret_cast<float*>(ws);
+  const int v4 = vec4_ok(enc, enc_dims) && vec4_ok(enc_dot, enc_dims) &&
+                 vec4_ok(g_enc, enc_dims) && vec4_ok(g_enc_dot, enc_dims);
+  SN_MLP16_DISPATCH(d_hidden, n_layers, kb0, {
+    constexpr size_t lds = Cfg<kH, kNH, kKB>::bwdad_lds;
+    allow_lds(mlp16_bwd_ad_kernel<kH, kNH, kKB>, lds);
+    mlp16_bwd_ad_kernel<kH, kNH, kKB><<<grid, 64 * kWaves, lds, s>>>(
+        x, enc, v, enc_dot, enc_dims, v4, n_cap, n_dev, mlp, beta, g_sdf, g_enc, g_enc_dot,
+        part);
+  });
+  if ((rc = check_launch("sn_mlp16_bwd_ad_enc"))) return rc;
   return mlp_reduce_rows(part, (int64_t)grid, np, g_mlp, accumulate, s);
 }
 

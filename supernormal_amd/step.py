@@ -97,8 +97,9 @@ class SyncFreeStep:
         if method not in ("dfd", "ad"):
             raise ValueError("gradient method must be 'dfd' or 'ad'")
         # 'ad': the rendered normal is the analytic x-gradient of the SDF
-        # (renderer.py:225-226), its loss differentiated twice; the patch points'
-        # MLP runs on the fp32 kernels (the forward-over-reverse backward is fp32)
+        # (renderer.py:225-226), its loss differentiated twice; with precision
+        # fp16 the patch points' forward, gradient and forward-over-reverse
+        # backward run on the fp16/bf16 matrix cores, else on the fp32 kernels
         self.ad = method == "ad"
         # forward gathers from an fp16 copy of the hash table, refreshed by each
         # Adam step (fp32 master table and gradients); default with precision fp16
@@ -229,7 +230,7 @@ class SyncFreeStep:
         hb = bucket_workspace_bytes(self.cfg.hash_args(self.cfg.grid["n_levels"]), self.c1)
         self.ws_hash = nb(hb) if hb else None
         cfg = self.cfg
-        if self.ad:
+        if self.ad and self.precision != "fp16":
             wsb = lib.sn_mlp_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers, 1)
         elif self.precision == "fp16":
             wsb = lib.sn_mlp16_bwd_workspace_bytes(self.c1, E, cfg.d_hidden, cfg.n_layers)
@@ -428,7 +429,7 @@ class SyncFreeStep:
              ptr(self.ns), self.normal_w, self.eik_w, self.mask_w, 1 if self.l1 else 0,
              ptr(self.g_sdf), None, ptr(self.g_grad) if ad else None, ptr(self.losses),
              ptr(self.partials), None, None, s)
-        # backward: MLP, hash grid (atomics straight into the flat gradient)
+        # backward: the MLP here; the hash grid in its own part (_part_hash)
         if ad:
             self._ad_backward(c1, mlp_d, s)
         elif self.precision == "fp16":
@@ -451,8 +452,9 @@ class SyncFreeStep:
 
     def _sdf_and_gradient(self, c1, mlp_d, s):
         """'ad': sdf and the analytic x-gradient of every patch point
-        (fields.py:107-119): MLP forward with d sdf / d h0, then the table's part
-        of the gradient by a gather."""
+        (fields.py:107-119): MLP forward with d sdf / d h0 (on the fp16 matrix
+        cores with precision fp16), then the table's part of the gradient by a
+        gather."""
         cfg = self.cfg
         params = self.net.encoding.params
         if self.table16 is not None:
@@ -461,7 +463,8 @@ class SyncFreeStep:
         else:
             call("sn_hashgrid_fwd_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
                  *self._hash_args(), ptr(self.levels_dev), ptr(self.enc), s)
-        call("sn_mlp_fwd", ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne), ptr(mlp_d),
+        fwd = "sn_mlp16_fwd_grad" if self.precision == "fp16" else "sn_mlp_fwd"
+        call(fwd, ptr(self.pts), ptr(self.enc), cfg.E, c1, ptr(self.ne), ptr(mlp_d),
              cfg.d_hidden, cfg.n_layers, cfg.beta, ptr(self.sdf), ptr(self.dh0), s)
         call("sn_hashgrid_dx_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
              ptr(self.table16), *self._hash_args(), ptr(self.levels_dev), ptr(self.dh0),
@@ -479,7 +482,8 @@ class SyncFreeStep:
         call("sn_hashgrid_bwd_bwd_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
              *self._hash_args(), ptr(self.levels_dev), None, ptr(self.g_grad),
              ptr(self.enc_dot), None, None, s)
-        call("sn_mlp_bwd_ad_enc", ptr(self.pts), ptr(self.enc), ptr(self.g_grad),
+        bwd = "sn_mlp16_bwd_ad_enc" if self.precision == "fp16" else "sn_mlp_bwd_ad_enc"
+        call(bwd, ptr(self.pts), ptr(self.enc), ptr(self.g_grad),
              ptr(self.enc_dot), cfg.E, c1, ptr(self.ne), ptr(mlp_d), cfg.d_hidden,
              cfg.n_layers, cfg.beta, ptr(self.g_sdf), ptr(self.g_enc), ptr(self.g_enc_dot),
              ptr(self.g_mlp), 0, ptr(self.ws_mlp), self.ws_mlp.numel(), s)
