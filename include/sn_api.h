@@ -638,6 +638,17 @@ int sn_hashgrid_bwd_bwd_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
                         const int32_t* n_active_dev,
                             const float* grad_enc, const float* g_gx, float* grad_grad_enc,
                             float* grad_params, float* grad_x, void* stream);
+/* The encoding's directional derivative along g_gx (sn_hashgrid_bwd_bwd_dev */
+/* with only grad_grad_enc: tcnn's backward_backward_input's dL/d(dL/dy)    */
+/* term, models/fields.py:107-119) from the fp32 table or, when params16 is */
+/* given, its fp16 copy (sn_hashgrid_fwd16_dev's); F = 2 with the levels a   */
+/* multiple of 4 gathers 4 levels at a time, other configs need params.     */
+int sn_hashgrid_tangent_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                            const float* params, const void* params16, int32_t n_levels,
+                            int32_t n_features, int32_t log2_hashmap_size,
+                            int32_t base_resolution, float per_level_scale,
+                            int32_t n_active_levels, const int32_t* n_active_dev,
+                            const float* g_gx, float* grad_grad_enc, void* stream);
 /* The table gradient of the analytic-gradient iteration in one pass: the   */
 /* first-order term of grad_enc (as sn_hashgrid_bwd_ws_dev) plus the double */
 /* backward's table term that sn_hashgrid_bwd_bwd_dev adds for              */

@@ -115,6 +115,7 @@ SIGNATURES = {
     "sn_hashgrid_bwd_workspace_bytes": (_i64, [_i32, _i32, _i32, _i32, _f32, _i32]),
     "sn_hashgrid_bwd_ws_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _sz, _p]),
     "sn_hashgrid_bwd_bwd_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _p, _p]),
+    "sn_hashgrid_tangent_dev": (C.c_int, [_p, _i64, _p, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p]),
     "sn_hashgrid_bwd_ad_ws_dev": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _i32, _f32, _i32, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "sn_occ_sample_workspace_bytes": (_sz, [_i32]),
     "sn_occ_sample_cells": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _sz, _p]),

@@ -233,6 +233,64 @@ hash_fwd_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __res
     hash_fwd_point<F, TP, kPair>(i, x, params, d, na, enc);
 }
 
+// The encoding's directional derivative along v (the tangent the double
+// backward of the analytic gradient needs, hash_bwd_bwd_kernel's grad_grad_enc):
+// out[l, f] = sum_c scale_l (v . d w_c / d frac) table_l[idx_c, f], from the
+// fp32 table or its fp16 copy.  F = 2, rows of 16 bytes: 4 levels per step,
+// their 32 corner gathers issued before any is used (as the forward).
+template <typename TP>
+__global__ void __launch_bounds__(256)
+hash_tangent_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
+                    const TP* __restrict__ params, HashGridDesc d, const float* __restrict__ gv,
+                    float* __restrict__ out) {
+  typedef TP pair_t __attribute__((ext_vector_type(2)));
+  const int64_t n = dev_count(n_cap, n_dev);
+  const int na = active_levels(d);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    const float v[3] = {gv[3 * i], gv[3 * i + 1], gv[3 * i + 2]};
+    float* o = out + i * (int64_t)(d.n_levels * 2);
+    for (int l0 = 0; l0 < d.n_levels; l0 += 4) {
+      float a[4][8];
+      pair_t t[4][8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (l0 + j < na) {  // uniform
+          uint32_t idx[8];
+          float w[8], fr[3], dw[8][3];
+          hg_level(p, d, l0 + j, idx, w, fr);
+          hg_dweights(fr, dw);
+          const float sc = d.scale[l0 + j];
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            a[j][c] = sc * (v[0] * dw[c][0] + v[1] * dw[c][1] + v[2] * dw[c][2]);
+          const pair_t* tab2 = reinterpret_cast<const pair_t*>(params + (int64_t)d.offset[l0 + j] * 2);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) t[j][c] = tab2[idx[c]];
+        }
+      }
+      float r[8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float s0 = 0.f, s1 = 0.f;
+        if (l0 + j < na) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            s0 += a[j][c] * (float)t[j][c].x;
+            s1 += a[j][c] * (float)t[j][c].y;
+          }
+        }
+        r[2 * j] = s0;
+        r[2 * j + 1] = s1;
+      }
+      float4* o4 = reinterpret_cast<float4*>(o + l0 * 2);
+      o4[0] = make_float4(r[0], r[1], r[2], r[3]);
+      o4[1] = make_float4(r[4], r[5], r[6], r[7]);
+    }
+  }
+}
+
 // Wavefront-aggregated atomic add of F values at table row idx: lanes that
 // target the same row sum their values first and one lane issues the atomics.
 // Every lane of the wavefront must call it (inactive lanes with active=false).
@@ -679,7 +737,7 @@ struct StagedRecord {
 // genc2 of the encoding's derivative along v, corner c of level l adds
 // scale_l (v . d w_c / d frac) genc2[i, l, f] to row idx_c, feature f.
 template <int F, bool kAd = false>
-__global__ void __launch_bounds__(kRecThreads)
+__global__ void __launch_bounds__(kRecThreads, 4)  // 4 wavefronts per SIMD: 2 workgroups per CU
 hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* __restrict__ n_dev,
                     const float* __restrict__ params, HashGridDesc d,
                     const float* __restrict__ genc, float* __restrict__ gparams,
@@ -1453,6 +1511,42 @@ int sn_hashgrid_bwd_bwd(const float* x, int64_t n, const float* params, int32_t 
                                  base_resolution, per_level_scale, n_active_levels, nullptr,
                                  grad_enc,
                                  g_gx, grad_grad_enc, grad_params, grad_x, stream);
+}
+
+int sn_hashgrid_tangent_dev(const float* x, int64_t n_cap, const int32_t* n_dev,
+                            const float* params, const void* params16, int32_t n_levels,
+                            int32_t n_features, int32_t log2_hashmap_size,
+                            int32_t base_resolution, float per_level_scale,
+                            int32_t n_active_levels, const int32_t* n_active_dev,
+                            const float* g_gx, float* grad_grad_enc, void* stream) {
+  clear_error();
+  HashGridDesc d;
+  int rc = hg_setup("sn_hashgrid_tangent", n_levels, n_features, log2_hashmap_size,
+                    base_resolution, per_level_scale, n_active_levels, &d);
+  if (rc) return rc;
+  d.n_active_dev = n_active_dev;
+  SN_REQUIRE(n_cap >= 0, "sn_hashgrid_tangent: n < 0");
+  if (n_cap == 0) return SN_OK;
+  SN_REQUIRE(x && (params || params16) && g_gx && grad_grad_enc,
+             "sn_hashgrid_tangent: null argument");
+  hipStream_t s = as_stream(stream);
+  const bool paired = n_features == 2 && n_levels % 4 == 0 &&
+                      (reinterpret_cast<uintptr_t>(grad_grad_enc) & 15) == 0;
+  if (!paired) {
+    SN_REQUIRE(params, "sn_hashgrid_tangent: this config needs the fp32 table");
+    SN_DISPATCH_F(n_features, hash_bwd_bwd_kernel<kF><<<grid_for(n_cap, 256, kHashBlocks), 256,
+                                                        0, s>>>(x, n_cap, n_dev, params, d,
+                                                                nullptr, g_gx, grad_grad_enc,
+                                                                nullptr, nullptr));
+    return check_launch("sn_hashgrid_tangent");
+  }
+  if (params16)
+    hash_tangent_kernel<_Float16><<<grid_for(n_cap, 256, kHashBlocks), 256, 0, s>>>(
+        x, n_cap, n_dev, static_cast<const _Float16*>(params16), d, g_gx, grad_grad_enc);
+  else
+    hash_tangent_kernel<float><<<grid_for(n_cap, 256, kHashBlocks), 256, 0, s>>>(
+        x, n_cap, n_dev, params, d, g_gx, grad_grad_enc);
+  return check_launch("sn_hashgrid_tangent");
 }
 
 }  // extern "C"

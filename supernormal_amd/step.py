@@ -479,9 +479,9 @@ class SyncFreeStep:
         params = self.net.encoding.params
         call("sn_fill_span_dev", ptr(self.g_grad), c1 * 3, ptr(self.ns), self.P * 3,
              ptr(self.ne), 3, 0.0, s)
-        call("sn_hashgrid_bwd_bwd_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
-             *self._hash_args(), ptr(self.levels_dev), None, ptr(self.g_grad),
-             ptr(self.enc_dot), None, None, s)
+        call("sn_hashgrid_tangent_dev", ptr(self.pts), c1, ptr(self.ne), ptr(params),
+             ptr(self.table16), *self._hash_args(), ptr(self.levels_dev), ptr(self.g_grad),
+             ptr(self.enc_dot), s)
         bwd = "sn_mlp16_bwd_ad_enc" if self.precision == "fp16" else "sn_mlp_bwd_ad_enc"
         call(bwd, ptr(self.pts), ptr(self.enc), ptr(self.g_grad),
              ptr(self.enc_dot), cfg.E, c1, ptr(self.ne), ptr(mlp_d), cfg.d_hidden,

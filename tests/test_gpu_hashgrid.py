@@ -299,6 +299,43 @@ def test_ad_table_gradient_one_pass(dev, with_ws):
         assert torch.equal(outs[1], outs[2])
 
 
+@pytest.mark.parametrize("half", [False, True])
+def test_tangent_vs_fp64(dev, half):
+    """sn_hashgrid_tangent_dev: the encoding's directional derivative along v
+    (d/de enc(x + e v) at e = 0) against fp64 autograd of the oracle; from the
+    fp32 table (1e-5 of max|ref|, and equal to sn_hashgrid_bwd_bwd_dev's
+    grad_grad_enc within fp32 reassociation) or its fp16 copy (2e-3)."""
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=31)
+    d = _desc(cfg)
+    rng = np.random.default_rng(32)
+    n = 3000
+    x = (rng.random((n, 3)) * 1.8 - 0.9).astype(np.float32)
+    v = rng.normal(size=(n, 3)).astype(np.float32)
+    x64 = torch.from_numpy(x).double().requires_grad_(True)
+    enc = encode(x64, e.params.detach().cpu().double(), d)
+    w = torch.zeros_like(enc, requires_grad=True)
+    gx, = torch.autograd.grad(enc, x64, grad_outputs=w, create_graph=True)
+    ref, = torch.autograd.grad((gx * torch.from_numpy(v).double()).sum(), w)
+    xt, vt = torch.from_numpy(x).to(dev), torch.from_numpy(v).to(dev)
+    args = (16, 2, 19, 16, 1.38, 16)
+    p16 = e.params.detach().half().contiguous() if half else None
+    out = torch.full((n, 32), -9.0, device=dev)
+    call("sn_hashgrid_tangent_dev", ptr(xt), n, None, ptr(e.params), ptr(p16), *args, None,
+         ptr(vt), ptr(out), stream_of(dev))
+    torch.cuda.synchronize()
+    scale = float(ref.abs().max())
+    err = float((out.cpu().double() - ref).abs().max()) / scale
+    print(f"\nhalf={half}: tangent max err {err:.2e} of max|ref|")
+    assert err <= (2e-3 if half else 1e-5)
+    if not half:
+        other = torch.empty_like(out)
+        call("sn_hashgrid_bwd_bwd_dev", ptr(xt), n, None, ptr(e.params), *args, None, None,
+             ptr(vt), ptr(other), None, None, stream_of(dev))
+        torch.testing.assert_close(out, other, rtol=1e-5, atol=1e-6 * scale)
+
+
 def test_full_size_properties(dev):
     """cfg2 table at the training batch size: partition of unity and linearity in
     the table (size-independent checks)."""
