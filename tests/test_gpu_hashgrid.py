@@ -336,6 +336,38 @@ def test_tangent_vs_fp64(dev, half):
         torch.testing.assert_close(out, other, rtol=1e-5, atol=1e-6 * scale)
 
 
+@pytest.mark.parametrize("half", [False, True])
+@pytest.mark.parametrize("n_active", [16, 9])
+def test_dx_vs_fp64(dev, half, n_active):
+    """sn_hashgrid_dx_dev: grad = dh0[:, :3] + sum_l dh0[:, 3 + l F + f] d enc_lf / dx
+    (the analytic gradient's chain through the table) against fp64 autograd of
+    the oracle, over the active levels; fp32 table 1e-5 of max|ref|, fp16 copy 2e-3."""
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=41)
+    d = _desc(cfg)
+    rng = np.random.default_rng(42)
+    n = 3000
+    x = (rng.random((n, 3)) * 1.8 - 0.9).astype(np.float32)
+    dh0 = rng.normal(size=(n, 35)).astype(np.float32)
+    x64 = torch.from_numpy(x).double().requires_grad_(True)
+    enc = encode(x64, e.params.detach().cpu().double(), d)
+    mask = torch.zeros(32, dtype=torch.float64)
+    mask[:2 * n_active] = 1.0
+    gx, = torch.autograd.grad((enc * torch.from_numpy(dh0[:, 3:]).double() * mask).sum(), x64)
+    ref = gx + torch.from_numpy(dh0[:, :3]).double()
+    xt, gt = torch.from_numpy(x).to(dev), torch.from_numpy(dh0).to(dev)
+    args = (16, 2, 19, 16, 1.38, n_active)
+    p16 = e.params.detach().half().contiguous() if half else None
+    out = torch.full((n, 3), -9.0, device=dev)
+    call("sn_hashgrid_dx_dev", ptr(xt), n, None, ptr(e.params), ptr(p16), *args, None, ptr(gt),
+         35, ptr(out), stream_of(dev))
+    torch.cuda.synchronize()
+    err = float((out.cpu().double() - ref).abs().max() / ref.abs().max())
+    print(f"\nhalf={half} active={n_active}: dx max err {err:.2e} of max|ref|")
+    assert err <= (2e-3 if half else 1e-5)
+
+
 def test_full_size_properties(dev):
     """cfg2 table at the training batch size: partition of unity and linearity in
     the table (size-independent checks)."""
