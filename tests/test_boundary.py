@@ -285,3 +285,36 @@ dataset {{
     V = torch.linalg.inv(ds.V_inverse_all[2])
     np.testing.assert_allclose(V[..., 1, :].numpy(), np.broadcast_to(poses[2][:3, 0], (H, W, 3)),
                                atol=1e-5)
+
+
+def test_profiling_reductions(tmp_path):
+    """The rocprofv3 CSV reductions behind profiles/: per-kernel means over
+    all and over the last dispatches, counter means, the probe means."""
+    from supernormal_amd import profiling
+    d = tmp_path / "run"
+    d.mkdir()
+    with open(d / "x_kernel_trace.csv", "w") as f:
+        f.write("Kernel_Name,Start_Timestamp,End_Timestamp,Queue_Id\n")
+        for i in range(10):
+            f.write(f"kA,{i * 1000},{i * 1000 + 100 + 10 * i},1\n")
+        f.write("kB,50000,52000,2\n")
+    out = tmp_path / "k.tsv"
+    profiling.kernel_means(str(d), str(out), last=2)
+    lines = out.read_text().splitlines()
+    assert lines[0].startswith("kernel\tcalls")
+    rows = {l.split("\t")[0]: l.split("\t") for l in lines[1:]}
+    assert rows["kA"][1] == "10" and rows["kA"][2] == "0.1" and rows["kA"][3] == "0.2"
+    assert rows["kB"][3] == "2.0"
+    with open(d / "x_counter_collection.csv", "w") as f:
+        f.write("Kernel_Name,Counter_Name,Dispatch_Id,Counter_Value\n")
+        for i in range(6):
+            f.write(f"kA,FETCH_SIZE,{i},{i}\n")
+    out2 = tmp_path / "c.tsv"
+    profiling.counter_means(str(d), str(out2), last=2)
+    assert out2.read_text().split("\t")[-1].strip() == "4.5"
+    out3 = tmp_path / "p.tsv"
+    profiling.probe_means(str(d), str(out3))
+    assert out3.read_text().split("\t")[-1].strip() == "3.0"
+    out4 = tmp_path / "t.tsv"
+    profiling.timeline(str(d), str(out4), last=3)
+    assert len(out4.read_text().splitlines()) == 3
