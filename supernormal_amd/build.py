@@ -29,6 +29,11 @@ ARCH = "gfx950"
 # Files that restate integer/ordering-sensitive reference semantics.
 NO_CONTRACT = {"ray_marching.hip", "occgrid.hip", "render_weight.hip", "render_step.hip", "sampler.hip",
                "step_kernels.hip"}
+# Files whose fp32 divisions and square roots are the hardware reciprocal / square root
+# (a few ulp) rather than the correctly rounded sequences: the fused rendering step,
+# compared with its fp64 oracle at 2e-3 (its ~280 divisions per lane were a third of
+# its VALU instructions; 202 -> 169 us per late-iteration launch).
+FAST_DIV = {"render_step.hip"}
 
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -52,6 +57,8 @@ def _compile(src: Path, obj: Path, verbose: bool) -> None:
     flags = list(COMMON_FLAGS)
     if src.name in NO_CONTRACT:
         flags.append("-ffp-contract=off")
+    if src.name in FAST_DIV:
+        flags.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
     cmd = [hipcc(), *flags, "-I", str(CSRC), "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -24,6 +24,11 @@
 //
 // Per-wave loss partials (normal, mask, eikonal, dL/dinv_s) go to `partials`
 // [n_waves, 4] and are summed in a fixed order by the caller's reduction.
+//
+// This file is built with -fno-hip-fp32-correctly-rounded-divide-sqrt: its fp32
+// divisions and square roots are the hardware reciprocal and square root (a few
+// ulp), not the correctly rounded sequences, which were a third of the sample-
+// parallel kernel's VALU instructions.
 #include "sn_common.h"
 
 namespace sn {
