@@ -572,6 +572,13 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    Records go to the list of their bucket (kBucketEntries consecutive rows
 //    of the level): ranks within the workgroup come from integer LDS atomics,
 //    room in each list from one global atomic per (workgroup, level, bucket).
+//    Per level: ranks, barrier, offsets (one wavefront's scan), barrier,
+//    staging in LDS by bucket while wavefront 0's reservation atomics are in
+//    flight, barrier, coalesced writes (measured per level and workgroup on
+//    the late iterations of a full training run, before the reservation
+//    overlapped the staging: hashing, grouping and ranks 3.6 us, waiting for
+//    the slowest wavefront 1.3 us, scan and reservation 1.5 us, staging
+//    1.3 us, writes 1.0 us).
 //    Records beyond a list's capacity are added with fp32 atomics instead.
 //    The lists share one pool; bucket_layout_kernel (before the records)
 //    gives each list 5/4 of the records it took in the previous call on the
@@ -783,7 +790,7 @@ __device__ __forceinline__ void group_sums(float (&a)[8 * F], int lane, int ld, 
 // so the writes of consecutive staging entries are consecutive in memory.
 struct StagedRecord {
   uint32_t key;  // row within the level
-  int32_t at;    // index in the pool, or -1: add with atomics (list full)
+  int32_t rank;  // position among the workgroup's records of its bucket
   float v[2];
 };
 
@@ -802,8 +809,9 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
                     const float* __restrict__ genc2 = nullptr) {
   __shared__ int32_t hist[kMaxBucketsPerLevel];   // records per bucket (this level)
   __shared__ int32_t lofs[kMaxBucketsPerLevel];   // their first staging slot
-  __shared__ int32_t rroom[kMaxBucketsPerLevel];  // list room left at the workgroup's share
-  __shared__ int32_t rbase[kMaxBucketsPerLevel];  // pool index of the share's first record
+  // per bucket: (list room left at the workgroup's share, pool index of the
+  // share's first record)
+  __shared__ int2 rres[kMaxBucketsPerLevel];
   __shared__ StagedRecord stage[kRecThreads * 8];
   __shared__ uint32_t lmax;
   __shared__ int32_t n_staged;
@@ -941,9 +949,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
       }
       __syncthreads();
-      // wavefront 0 alone touches the counts here: staging offsets (exclusive
-      // scan), room in each bucket's list (one global atomic per non-empty
-      // bucket), counts back to zero for the next level
+      // wavefront 0 alone: staging offsets (exclusive scan of the counts)
       if (tid < 64) {
         const int per = (nb + 63) / 64;  // buckets per lane, in order
         int sum = 0;
@@ -961,17 +967,8 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         for (int q = 0; q < per; ++q) {
           const int b = tid * per + q;
           if (b < nb) {
-            const int32_t c = hist[b];
             lofs[b] = run;
-            run += c;
-            if (c) {
-              // relative position in the list; the pool index only while in room
-              const int32_t r0 = atomicAdd(&pool.cursor[gb0 + b], c);
-              const int32_t room = pool.bcap[gb0 + b] - r0;
-              rroom[b] = room;
-              rbase[b] = room > 0 ? pool.start[gb0 + b] + r0 : 0;
-            }
-            hist[b] = 0;
+            run += hist[b];
           }
         }
         if (tid == 63) n_staged = incl;
@@ -981,16 +978,52 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         lmax = 0u;
       }
       __syncthreads();
-      if (has) {
+      // every wavefront stages its records (the staging needs only the
+      // offsets); wavefront 0 also takes room in each bucket's list (one global
+      // atomic per non-empty bucket: position relative to the list, the pool
+      // index only while in room) and sets the counts back to zero, with the
+      // atomics issued before its own staging when a lane holds one bucket, so
+      // that their round trip overlaps the staging
+      auto stage_own = [&]() {
+        if (has) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int b = (int)(idx[c] >> kBucketShift);
-          StagedRecord& e = stage[lofs[b] + rank[c]];
-          e.key = idx[c];
-          e.at = rank[c] < rroom[b] ? rbase[b] + rank[c] : -1;
+          for (int c = 0; c < 8; ++c) {
+            StagedRecord& e = stage[lofs[idx[c] >> kBucketShift] + rank[c]];
+            e.key = idx[c];
+            e.rank = rank[c];
 #pragma unroll
-          for (int f = 0; f < F; ++f) e.v[f] = rv[c][f];
+            for (int f = 0; f < F; ++f) e.v[f] = rv[c][f];
+          }
         }
+      };
+      if (tid < 64) {
+        if (nb <= 64) {
+          const int32_t c = tid < nb ? hist[tid] : 0;
+          const int32_t r0 = c ? atomicAdd(&pool.cursor[gb0 + tid], c) : 0;
+          stage_own();
+          if (c) {
+            const int32_t room = pool.bcap[gb0 + tid] - r0;
+            rres[tid] = make_int2(room, room > 0 ? pool.start[gb0 + tid] + r0 : 0);
+          }
+          if (tid < nb) hist[tid] = 0;
+        } else {
+          stage_own();
+          const int per = (nb + 63) / 64;
+          for (int q = 0; q < per; ++q) {
+            const int b = tid * per + q;
+            if (b < nb) {
+              const int32_t c = hist[b];
+              if (c) {
+                const int32_t r0 = atomicAdd(&pool.cursor[gb0 + b], c);
+                const int32_t room = pool.bcap[gb0 + b] - r0;
+                rres[b] = make_int2(room, room > 0 ? pool.start[gb0 + b] + r0 : 0);
+              }
+              hist[b] = 0;
+            }
+          }
+        }
+      } else {
+        stage_own();
       }
       __syncthreads();
       {
@@ -998,10 +1031,12 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         const int ns = n_staged;
         for (int q = tid; q < ns; q += kRecThreads) {
           const StagedRecord e = stage[q];
-          if (e.at >= 0) {
-            pool.row[e.at] = (uint16_t)(e.key & (kBucketEntries - 1));
+          const int2 rr = rres[e.key >> kBucketShift];
+          if (e.rank < rr.x) {
+            const int64_t at = rr.y + e.rank;
+            pool.row[at] = (uint16_t)(e.key & (kBucketEntries - 1));
 #pragma unroll
-            for (int f = 0; f < F; ++f) pool.val[(int64_t)e.at * F + f] = e.v[f];
+            for (int f = 0; f < F; ++f) pool.val[at * F + f] = e.v[f];
           } else {
 #pragma unroll
             for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)e.key * F + f], e.v[f]);

@@ -97,17 +97,20 @@ def test_backward_aggregation_matches_plain_atomics(name, dev):
         torch.testing.assert_close(gxs[k], gxs[2], rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("log2_T", [19, 21])
 @pytest.mark.parametrize("records", [65536, 7])
-def test_bucket_backward_matches_atomics(records, dev):
+def test_bucket_backward_matches_atomics(records, log2_T, dev):
     """sn_hashgrid_bwd_ws_dev (bucket records, then per-bucket fixed-point
     sums) equals the atomic path within fp32 reassociation, adding onto a
     non-zero gradient and giving the same x-gradient, and repeats bit for bit;
     with 7 records per bucket almost every record takes the overflow atomics.
     Only the first n_dev of n_cap points count.  The workspace starts as
-    garbage and is reused by a second call."""
+    garbage and is reused by a second call.  T = 2^19 gives 64 buckets per
+    hashed level, T = 2^21 gives 256 (several buckets per lane of the
+    record kernel's reservation)."""
     from supernormal_amd import _lib
     from supernormal_amd._lib import call, ptr, stream_of
-    cfg = CFGS["cfg2"]
+    cfg = dict(CFGS["cfg2"], log2_hashmap_size=log2_T)
     e = _enc(cfg, dev, seed=7)
     g = torch.Generator().manual_seed(8)
     centers = torch.rand(200, 3, generator=g) * 1.6 - 0.8
@@ -116,7 +119,7 @@ def test_bucket_backward_matches_atomics(records, dev):
     n_cap, n_use = x.shape[0], 35000
     n_dev = torch.tensor([n_use], dtype=torch.int32, device=dev)
     genc = torch.randn(n_cap, e.n_output_dims, generator=g).to(dev)
-    gargs = (16, 2, 19, 16, 1.38, 16)
+    gargs = (16, 2, log2_T, 16, 1.38, 16)
     lib = _lib.lib()
     nb = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], records)
     assert nb > 0
