@@ -892,8 +892,10 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       bool has = act;
       uint64_t pend = __ballot(act);
       {
-        const uint32_t p0 = __shfl_up(cell[0], 1, 64), p1 = __shfl_up(cell[1], 1, 64),
-                       p2 = __shfl_up(cell[2], 1, 64);
+        // the previous lane's cell by a DPP wave shift (wave_shr:1), not an
+        // LDS permute; lane 0 reads 0 and is excluded below
+        const uint32_t p0 = dpp_u<0x138>(cell[0]), p1 = dpp_u<0x138>(cell[1]),
+                       p2 = dpp_u<0x138>(cell[2]);
         const bool same_prev = lane > 0 && ((pend >> (lane - 1)) & 1ull) && act &&
                                cell[0] == p0 && cell[1] == p1 && cell[2] == p2;
         if (__ballot(same_prev) == 0ull) pend = 0ull;  // uniform
@@ -963,7 +965,13 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
         for (int k = 0; k < 3; ++k) dx[k] += d.scale[l] * acc[k];
       }
       __syncthreads();
-      // wavefront 0 alone: staging offsets (exclusive scan of the counts)
+      // wavefront 0 alone: staging offsets (exclusive scan of the counts);
+      // with one bucket per lane (nb <= 64) it first issues the lane's list
+      // reservation (one global atomic per non-empty bucket: position
+      // relative to the list), whose result it needs only after the scan,
+      // the barrier below and its own staging: the barrier waits for LDS
+      // operations only, so the round trip overlaps all three
+      int32_t res_c = 0, res_r0 = 0;
       if (tid < 64) {
         const int per = (nb + 63) / 64;  // buckets per lane, in order
         int sum = 0;
@@ -971,13 +979,20 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
           const int b = tid * per + q;
           if (b < nb) sum += hist[b];
         }
-        int incl = sum;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int y = __shfl_up(incl, off, 64);
-          if (lane >= off) incl += y;
+        if (nb <= 64) {
+          res_c = sum;
+          if (res_c) res_r0 = atomicAdd(&pool.cursor[gb0 + tid], res_c);
         }
-        int run = incl - sum;
+        // inclusive scan across the wavefront on DPP lane moves (row shifts,
+        // then the rows' totals broadcast upwards), no LDS round trips
+        uint32_t incl = (uint32_t)sum;
+        incl += dpp_u<0x111>(incl);
+        incl += dpp_u<0x112>(incl);
+        incl += dpp_u<0x114>(incl);
+        incl += dpp_u<0x118>(incl);
+        incl += dpp_u<0x142, 0xa>(incl);
+        incl += dpp_u<0x143, 0xc>(incl);
+        int run = (int)incl - sum;
         for (int q = 0; q < per; ++q) {
           const int b = tid * per + q;
           if (b < nb) {
@@ -985,7 +1000,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
             run += hist[b];
           }
         }
-        if (tid == 63) n_staged = incl;
+        if (tid == 63) n_staged = (int)incl;
       }
       if (tid == 0) {
         atomicMax(&pool.maxbits[l], lmax);
@@ -993,11 +1008,9 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       }
       __syncthreads();
       // every wavefront stages its records (the staging needs only the
-      // offsets); wavefront 0 also takes room in each bucket's list (one global
-      // atomic per non-empty bucket: position relative to the list, the pool
-      // index only while in room) and sets the counts back to zero, with the
-      // atomics issued before its own staging when a lane holds one bucket, so
-      // that their round trip overlaps the staging
+      // offsets); wavefront 0 then turns the reservations into (room, pool
+      // index while in room) per bucket -- with several buckets per lane it
+      // takes them here -- and sets the counts back to zero
       auto stage_own = [&]() {
         if (has) {
 #pragma unroll
@@ -1012,12 +1025,10 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       };
       if (tid < 64) {
         if (nb <= 64) {
-          const int32_t c = tid < nb ? hist[tid] : 0;
-          const int32_t r0 = c ? atomicAdd(&pool.cursor[gb0 + tid], c) : 0;
           stage_own();
-          if (c) {
-            const int32_t room = pool.bcap[gb0 + tid] - r0;
-            rres[tid] = make_int2(room, room > 0 ? pool.start[gb0 + tid] + r0 : 0);
+          if (res_c) {
+            const int32_t room = pool.bcap[gb0 + tid] - res_r0;
+            rres[tid] = make_int2(room, room > 0 ? pool.start[gb0 + tid] + res_r0 : 0);
           }
           if (tid < nb) hist[tid] = 0;
         } else {

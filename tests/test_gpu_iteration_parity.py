@@ -20,6 +20,12 @@ up along it), and the oracle then continues from the device's decision.
 Within stated tolerances (fp32 / fp16 MLP with fp16 table copy):
   the normal, mask and eikonal losses (relative);
   the gradient of every parameter tensor: relative 2-norm of the difference.
+Measured relative gradient errors: fp32 1.5e-5 to 1e-4 per tensor; fp16
+4e-2 to 1.0e-1 per MLP tensor and 3.8e-2 to 4.5e-2 for the table over two
+trained states (the fp16 forward's 11-bit and the bf16 backward's 8-bit
+significands, summed over ~1.2 M points whose terms largely cancel in the
+weight gradients), hence 2e-1 for the fp16 MLP tensors; a wrong term moves a
+tensor's gradient by O(1).
 """
 import math
 
@@ -32,7 +38,7 @@ pytestmark = pytest.mark.gpu
 # (keep band in |ln T - ln 1e-8|, loss rtol, gradient relative-norm tolerance
 #  for the hash table and for the MLP / variance tensors)
 TOL = {"fp32": dict(band=0.05, loss=2e-3, table=2e-2, mlp=2e-2),
-       "fp16": dict(band=4.0, loss=3e-2, table=1e-1, mlp=1e-1)}
+       "fp16": dict(band=4.0, loss=3e-2, table=1e-1, mlp=2e-1)}
 
 
 def _trained_runner(dev, precision):
