@@ -1054,9 +1054,7 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
       {
         float* tg = gparams + (int64_t)d.offset[l] * F;
         const int ns = n_staged;
-        for (int q = tid; q < ns; q += kRecThreads) {
-          const StagedRecord e = stage[q];
-          const int2 rr = rres[e.key >> kBucketShift];
+        auto put = [&](const StagedRecord& e, int2 rr) {
           if (e.rank < rr.x) {
             const int64_t at = rr.y + e.rank;
             pool.row[at] = (uint16_t)(e.key & (kBucketEntries - 1));
@@ -1066,6 +1064,19 @@ hash_bwd_rec_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 #pragma unroll
             for (int f = 0; f < F; ++f) atomicAdd(&tg[(int64_t)e.key * F + f], e.v[f]);
           }
+        };
+        // two records per lane and step, both read from LDS before either is
+        // written out (record kernel 527 -> 516 us per late launch)
+        int q = tid;
+        for (; q + kRecThreads < ns; q += 2 * kRecThreads) {
+          const StagedRecord e0 = stage[q], e1 = stage[q + kRecThreads];
+          const int2 r0 = rres[e0.key >> kBucketShift], r1 = rres[e1.key >> kBucketShift];
+          put(e0, r0);
+          put(e1, r1);
+        }
+        if (q < ns) {
+          const StagedRecord e = stage[q];
+          put(e, rres[e.key >> kBucketShift]);
         }
       }
     }
