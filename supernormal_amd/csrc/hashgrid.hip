@@ -645,8 +645,11 @@ constexpr int32_t kLayoutMagic = 0x4c594f54;
 // Largest list unless the pool's even split is larger: the sum kernel takes
 // a list in one workgroup, whose integer LDS atomics (~5e9 lane ops/s per CU)
 // make a list of n records last at least n * F / 5e9 s; records beyond it
-// take the atomic path.
-constexpr int64_t kMaxList = 131072;
+// take the atomic path.  2^18: the coarsest levels' one or two buckets take
+// ~230-270 k records per late call of a 1.2 M-point backward (2^17 sent 14 %
+// of them to the atomics; the longest list, ~100 us in the sum kernel, runs
+// beside the other buckets' workgroups).
+constexpr int64_t kMaxList = 262144;
 constexpr int64_t kMaxPoolRecords = 1ll << 30;  // keeps the layout's products in int64
 constexpr int kLayoutThreads = 1024;
 constexpr int kMaxBuckets = kMaxLevels * kMaxBucketsPerLevel;

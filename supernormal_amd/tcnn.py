@@ -60,10 +60,11 @@ def bucket_workspace_bytes(gargs, n_points):
     record for every corner of every level of every point (8 per point and
     level), at most ~4 GiB; the kernels share it out over the buckets by the
     counts of the previous call on the same workspace, and records beyond a
-    bucket's share take the atomic path.  Lists longer than 131072 records
-    need a pool whose even split is larger: 196608 per bucket (measured on a
-    full training run: the late launches, up to ~180k records per fine-level
-    list, 6 % faster than with 131072; no change with 262144)."""
+    bucket's share take the atomic path.  The pool holds 196608 records per
+    bucket (measured on a full training run: the late launches, up to ~180k
+    records per fine-level list, 6 % faster than with 131072; no change with
+    262144); a list may take up to 262144 of them (the coarsest levels' one
+    or two buckets take ~230-270k records per late launch)."""
     n_levels, F, log2T, base, pls = gargs[:5]
     lib = _lib.lib()
     one = lib.sn_hashgrid_bwd_workspace_bytes(n_levels, F, log2T, base, pls, 1)
