@@ -80,10 +80,12 @@ def _sdf_values(sc, g):
 @pytest.mark.parametrize("method,loss_type,Ph,Pw,max_cnt", [
     ("dfd", "l2", 3, 3, 40), ("ad", "l2", 3, 3, 40), ("dfd", "l1", 3, 3, 40),
     ("dfd", "l2", 5, 5, 40), ("dfd", "l2", 2, 4, 40), ("dfd", "l2", 3, 3, 150),
-    ("ad", "l1", 3, 3, 150), ("dfd", "l2", 2, 4, 130)])
+    ("ad", "l1", 3, 3, 150), ("dfd", "l2", 2, 4, 130), ("dfd", "l2", 3, 3, 600),
+    ("ad", "l2", 2, 4, 600)])
 def test_render_step_vs_fp64(method, loss_type, Ph, Pw, max_cnt, impl, dev):
     """impl 1: sample-parallel kernel (3x3 and 2x4 patches; wavefront scans,
-    several 64-sample chunks when max_cnt > 64); impl 0: ray-parallel kernel."""
+    several 64-sample chunks with carries between them when max_cnt > 64, up
+    to 10 at 600); impl 0: ray-parallel kernel."""
     from supernormal_amd import _lib
     from supernormal_amd._lib import call, ptr, stream_of
     _lib.lib().sn_render_step_set_impl(impl)
