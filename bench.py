@@ -597,8 +597,9 @@ def main(argv=None):
                             "ms_per_step": round(resad["ms_per_step"], 4),
                             "full_schedule": round(resad["full_schedule"], 2),
                             "note": "same schedule and seed, rendered normal = analytic "
-                                    "x-gradient of the SDF (renderer.py:225-226), double "
-                                    "backward on the fp32 MLP kernels"}
+                                    "x-gradient of the SDF (renderer.py:225-226), its double "
+                                    "backward forward-over-reverse on the fp16/bf16 MLP "
+                                    "kernels (fp32 ones with --precision fp32)"}
         if not a.no_accuracy:
             out["method_ad"]["accuracy"] = accuracy(rad)
         log(f"method ad leg: {resad['value']:.1f} it/s")

@@ -33,9 +33,10 @@ bandwidth increase changes no launch.  While a march is pending, the buffers it 
 (inputs, counts, packed0, n0, ri0/t00/t10) already hold the next iteration's.
 
 Both gradient methods run here: 'dfd' (finite differences of the SDF at the
-patch points) and 'ad' (the analytic x-gradient and its double backward, on
-the fp32 MLP kernels).  max_samples_per_ray must be set (it bounds the
-capacity).
+patch points) and 'ad' (the analytic x-gradient and its double backward,
+forward-over-reverse, on the fp16/bf16 MLP kernels when the precision is
+fp16, on the fp32 ones otherwise).  max_samples_per_ray must be set (it
+bounds the capacity).
 """
 from __future__ import annotations
 
