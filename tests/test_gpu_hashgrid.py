@@ -148,6 +148,45 @@ def test_bucket_backward_matches_atomics(records, log2_T, dev):
              ptr(genc), ptr(g_b), None, ptr(ws), 64, s)
 
 
+def test_bucket_backward_propagates_nan_inf(dev):
+    """A NaN and an inf in dL/denc reach the table gradient through the bucket
+    path as through the fp32 atomic path: the level maximum is taken over the
+    bits of |value| (a NaN's bits lie above inf's), so both levels fall back to
+    fp32 sums, the rows the two points touch end NaN / +inf in both paths, and
+    every other row agrees within fp32 reassociation."""
+    from supernormal_amd import _lib
+    from supernormal_amd._lib import call, ptr, stream_of
+    cfg = CFGS["cfg2"]
+    e = _enc(cfg, dev, seed=21)
+    g = torch.Generator().manual_seed(22)
+    x = (torch.rand(20000, 3, generator=g) * 1.6 - 0.8).contiguous().to(dev)
+    n = x.shape[0]
+    genc = torch.randn(n, e.n_output_dims, generator=g)
+    genc[5, 2 * 3 + 0] = float("nan")    # level 3 (dense), feature 0
+    genc[7, 2 * 10 + 1] = float("inf")   # level 10 (hashed), feature 1
+    genc = genc.to(dev)
+    gargs = (16, 2, 19, 16, 1.38, 16)
+    lib = _lib.lib()
+    nb = lib.sn_hashgrid_bwd_workspace_bytes(*gargs[:5], 65536)
+    ws = torch.zeros((nb,), dtype=torch.uint8, device=dev)
+    s = stream_of(dev)
+    g_ref = torch.zeros_like(e.params)
+    call("sn_hashgrid_bwd_dev", ptr(x), n, None, ptr(e.params), *gargs, None, ptr(genc),
+         ptr(g_ref), None, s)
+    g_b = torch.zeros_like(e.params)
+    call("sn_hashgrid_bwd_ws_dev", ptr(x), n, None, ptr(e.params), *gargs, None, ptr(genc),
+         ptr(g_b), None, ptr(ws), nb, s)
+    torch.cuda.synchronize()
+    nan_ref, inf_ref = torch.isnan(g_ref), torch.isinf(g_ref)
+    assert int(nan_ref.sum()) > 0 and int(inf_ref.sum()) > 0
+    assert torch.equal(torch.isnan(g_b), nan_ref)
+    assert torch.equal(torch.isinf(g_b), inf_ref)
+    assert torch.equal(torch.sign(g_b[inf_ref]), torch.sign(g_ref[inf_ref]))
+    fin = ~(nan_ref | inf_ref)
+    scale = float(g_ref[fin].abs().max())
+    torch.testing.assert_close(g_b[fin], g_ref[fin], rtol=1e-4, atol=1e-5 * scale)
+
+
 def test_bucket_pool_follows_previous_counts(dev):
     """The record pool is shared out by the counts of the previous call on the
     same workspace.  12000 uniform points leave ~96k records in the single
