@@ -614,7 +614,7 @@ int sn_hashgrid_dx_dev(const float* x, int64_t n_cap, const int32_t* n_dev, cons
 /* the bytes for records_per_bucket times the bucket count; -1: unsupported */
 /* config).  Each call shares the pool out by the counts the previous call  */
 /* on the same workspace left in it (5/4 of each list's count plus a floor, */
-/* at most 131072 records or the even split if larger; scaled down to the   */
+/* at most 262144 records or the even split if larger; scaled down to the   */
 /* pool, or sharing what they leave of it evenly up to the same cap; the    */
 /* even split for a workspace no call has used): the                        */
 /* workspace carries this hint between calls, any content is valid input,  */
