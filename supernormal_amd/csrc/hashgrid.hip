@@ -607,7 +607,8 @@ constexpr int kMaxBucketsPerLevel = 512;
 // (group_sums; measured on the late iterations of a full training run, ms
 // per hash-grid backward: a minimum of 3 / 4 / 5 / 6 / 8 lanes -> 1.07 /
 // 1.05 / 1.036 / 1.03 / 1.045: smaller groups cost more in cross-lane sums
-// than they save in records); the search for groups (one leader per step,
+// than they save in records; with the cheaper ranks and staging of the
+// current record kernel 5 / 6 / 8 -> 0.897 / 0.901 / 0.901 ms, no difference); the search for groups (one leader per step,
 // lowest pending lane first) stops after kMaxSingles leaders in a row with a
 // smaller group (3 / 6 / 10 -> 1.05 / 1.03 / 1.044 ms), and is skipped on
 // levels where no lane shares the cell of the lane before it (the fine
