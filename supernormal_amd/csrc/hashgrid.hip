@@ -572,13 +572,15 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    Records go to the list of their bucket (kBucketEntries consecutive rows
 //    of the level): ranks within the workgroup come from integer LDS atomics,
 //    room in each list from one global atomic per (workgroup, level, bucket).
-//    Per level: ranks, barrier, offsets (one wavefront's scan), barrier,
-//    staging in LDS by bucket while wavefront 0's reservation atomics are in
-//    flight, barrier, coalesced writes (measured per level and workgroup on
-//    the late iterations of a full training run, before the reservation
-//    overlapped the staging: hashing, grouping and ranks 3.6 us, waiting for
-//    the slowest wavefront 1.3 us, scan and reservation 1.5 us, staging
-//    1.3 us, writes 1.0 us).
+//    Per level: ranks, barrier, offsets (one wavefront's scan, its
+//    reservation atomics issued first), barrier, staging in LDS by bucket
+//    while those atomics are in flight, barrier, coalesced writes (measured
+//    per level and workgroup on the late iterations of a full training run,
+//    with the reservation already overlapping the staging but before the
+//    per-wavefront level maximum and the DPP scan: hashing and loads 1.1 us,
+//    group search 1.6 us, ranks 1.35 us, waiting for the slowest wavefront
+//    1.4 us, scan 0.6 us, staging 0.6 us (wavefront 0 with the reservation
+//    1.4 us), writes 1.1 us).
 //    Records beyond a list's capacity are added with fp32 atomics instead.
 //    The lists share one pool; bucket_layout_kernel (before the records)
 //    gives each list 5/4 of the records it took in the previous call on the
@@ -590,8 +592,9 @@ hash_bwd_lds_kernel(const float* __restrict__ x, int64_t n_cap, const int32_t* _
 //    a pool of at least 5/4 of a call's records holds every list of a call
 //    that repeats the previous one's counts: each list's share is then at
 //    least (5/4 c) * pool / (5/4 records + pool / 4) >= c.
-//    The largest |value| of each level is kept (integer atomic max on the
-//    bits of a non-negative float).
+//    The largest |value| of each level is kept (integer max on the bits of a
+//    non-negative float: per wavefront by DPP, then one LDS atomic per
+//    wavefront and one global atomic per workgroup).
 // 2. hash_bwd_bucket_sum_kernel: one workgroup per bucket sums its records in
 //    LDS as 64-bit fixed point (integer LDS atomics; scale 2^K with
 //    K = 61 - ceil(log2(count * max|value|)), so no sum can overflow and each
