@@ -627,7 +627,9 @@ constexpr int kMaxSingles = SN_MAX_SINGLES;
 #ifndef SN_REC_THREADS
 #define SN_REC_THREADS 512
 #endif
-constexpr int kRecThreads = SN_REC_THREADS;  // points (lanes) per workgroup of the record kernel
+// points (lanes) per workgroup of the record kernel (the backward per late
+// launch of a full training run: 0.90 ms at 512, 1.22 ms at 256)
+constexpr int kRecThreads = SN_REC_THREADS;
 
 // Buckets of each level: level l owns global buckets [base[l], base[l+1]).
 struct BucketDesc {
