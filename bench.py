@@ -97,9 +97,10 @@ def hash_fwd_bytes(n, n_active, grid, table_bytes=4):
 
 # HBM bytes of a late hash-grid backward launch on this workload (all 16 levels
 # active), per point: two rocprofv3 counter runs over --traffic-probe (5 launches
-# on the last iteration's points), FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 -- the
-# factors are what a 64 MiB copy reads on each counter.
-HASH_BWD_TRAFFIC_PER_POINT = {"bytes_per_point": 2277261689 / 1229823, "probe_points": 1229823,
+# on the last iteration's points: 1225053 in the FETCH_SIZE run, 1225593 in the
+# WRITE_SIZE run), FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 -- the factors are what a
+# 64 MiB copy reads on each counter: 1.4195e9 / 1225053 + 0.8553e9 / 1225593.
+HASH_BWD_TRAFFIC_PER_POINT = {"bytes_per_point": 1856.6, "probe_points": 1225053,
                               "active_levels": 16}
 
 
